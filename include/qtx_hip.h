@@ -1,0 +1,206 @@
+/*
+ * qtx_hip.h — C ABI of the MI355X (gfx950) VMC-step library (libqtxhip.so).
+ *
+ * Every entry point:
+ *   - takes device pointers (unless stated otherwise), plain sizes and a HIP
+ *     stream handle passed as `void*` (NULL = the default stream);
+ *   - enqueues its work on that stream and returns 0 on success or a negative
+ *     QTX_E* code; `qtx_last_error()` returns the thread-local message;
+ *   - is reentrant: lattices, operators, models and random seeds are explicit
+ *     arguments (the reference keeps them in process globals).
+ *
+ * Spins are int8 +-1, row-major (ns, N).  Log-amplitudes are float64.  The
+ * sign of psi is +1 for both RBM_Dense (cosh > 0) and ResConv (exp head).
+ * Parameter vectors are in the reference's ravel_pytree order.
+ */
+#ifndef QTX_HIP_H
+#define QTX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define QTX_OK 0
+#define QTX_EINVAL (-1)   /* bad argument (shape, size, enum, null pointer) */
+#define QTX_EHIP (-2)     /* HIP runtime error */
+#define QTX_ESOLVER (-3)  /* rocSOLVER/rocBLAS failure or non-positive pivot */
+#define QTX_EWORKSPACE (-4) /* workspace too small */
+
+/* dtypes */
+#define QTX_F32 0
+#define QTX_F64 1
+
+/* proposal kinds */
+#define QTX_LOCALFLIP 0
+#define QTX_EXCHANGE 1
+
+/* Gram precision for the SR/MinSR solve */
+#define QTX_GRAM_FP64 0   /* FP64 products on FP64 MFMA                     */
+#define QTX_GRAM_FP32 1   /* FP32 products on FP32 MFMA                     */
+#define QTX_GRAM_BF16X3 2 /* split-bf16: hi*hi + hi*lo + lo*hi, FP32 accum. */
+
+/* solver modes */
+#define QTX_SOLVE_CHOL_SHIFT 0 /* (T + shift I) y = b by Cholesky           */
+#define QTX_SOLVE_EIGH_PINV 1  /* soft pseudo-inverse from eigh(T)          */
+
+/* ResConv forward precision */
+#define QTX_CONV_FP32 0   /* FP32 products on FP32 MFMA */
+#define QTX_CONV_BF16X3 1 /* split-bf16 products on bf16 MFMA, FP32 accum. */
+
+const char* qtx_last_error(void);
+int qtx_device_sync(void);
+
+/* ---------------------------------------------------------------- streams
+ * Philox4x32-10 blocks: words[(t * nchains + c) * 4 + w] for step t in
+ * [0, nsteps), chain c in [0, nchains); counter (chain0 + c, step0 + t, tag, 0),
+ * key (seed & 0xffffffff, seed >> 32).
+ * Replaces the host key splitting of quantax/global_defs.py:75-91 and the
+ * draws of sampler/common_samplers.py:31,65-73 / sampler/metropolis.py:303. */
+int qtx_philox_fill(void* stream, uint64_t seed, uint32_t tag, int64_t step0, int64_t nsteps,
+                    int64_t chain0, int64_t nchains, uint32_t* words);
+
+/* Initial chains: uniform +-1 (nup < 0) or a uniform permutation of nup up
+ * spins (Fisher-Yates on Philox draws).  Replaces utils/basis.py:160-219. */
+int qtx_init_spins(void* stream, uint64_t seed, int64_t nchains, int32_t N, int32_t nup, int8_t* s);
+
+/* ---------------------------------------------------------------- operator
+ * Compiled spin Hamiltonian (tables are device arrays).  Replaces the
+ * jax_op_list of quantax/operator/operator.py:219-236.
+ * diag term t : kind 0 -> c*s[a]*s[b], 1 -> c*s[a], 2 -> c.
+ * candidate k : flips site a (and b if b >= 0); valid iff (ra == 0 || s[a] == ra)
+ *               and (b < 0 || rb == 0 || s[b] == rb); matrix element c. */
+typedef struct qtx_ham {
+  int32_t N;
+  int32_t ndiag;
+  const double* diag_c;
+  const int32_t* diag_a;
+  const int32_t* diag_b;
+  const int32_t* diag_kind;
+  int32_t noff;
+  const int32_t* off_a;
+  const int32_t* off_b;
+  const int8_t* off_ra;
+  const int8_t* off_rb;
+  const double* off_c;
+} qtx_ham;
+
+/* Diagonal part of H on each sample (operator.py:81-93). */
+int qtx_ham_diag(void* stream, const qtx_ham* H, const int8_t* s, int64_t ns, double* diag);
+/* Number of valid connected configurations per sample (operator.py:122-141). */
+int qtx_conn_count(void* stream, const qtx_ham* H, const int8_t* s, int64_t ns, int32_t* count);
+/* Compacted connected configurations, sample-major, candidate order, at the
+ * exclusive-scan offsets of qtx_conn_count (operator.py:96-119, 144-165).
+ * Outputs: segment[K], cand[K], s_conn[K*N], h_conn[K]. */
+int qtx_conn_list(void* stream, const qtx_ham* H, const int8_t* s, int64_t ns, const int64_t* offsets,
+                  int32_t* segment, int32_t* cand, int8_t* s_conn, double* h_conn);
+
+/* ---------------------------------------------------------------- RBM_Dense
+ * Packed parameters: W transposed to (N, M) then b (M), in `dtype`.
+ * Build it from the ravel_pytree vector [W (M, N) row-major, b (M)] with
+ * qtx_rbm_pack (model/shallow_nets.py:35-126).  M <= 1024, N <= 1024. */
+int qtx_rbm_pack(void* stream, int dtype, const void* theta, int32_t M, int32_t N, void* packed);
+
+/* log|psi| = sum_i log cosh(h_i), h = W s + b; optionally h (ns, M) in dtype.
+ * Replaces Variational.__call__ / init_internal for SingleDense
+ * (state/variational.py:325-356, shallow_nets.py:80-85). */
+int qtx_rbm_logpsi(void* stream, int dtype, const void* packed, int32_t M, int32_t N, const int8_t* s,
+                   int64_t ns, double* logabs, void* h_out);
+
+/* nsteps Metropolis steps for all ns chains with h resident on chip.
+ * words: (nsteps, ns, 4) Philox blocks; nbr (N, width) -1-padded neighbour
+ * table (EXCHANGE only); hopping = +1 or -1 species moved by EXCHANGE.
+ * In/out: s.  Out: logabs (local-update value).  n_accept / n_near (ns,)
+ * are incremented (acceptances; decisions with |d - log1p(-u)| < tie_tol).
+ * Replaces Metropolis._partial_sweep/_single_sweep/_update with LocalFlip or
+ * SpinExchange proposals and SingleDense.ref_forward (metropolis.py:246-322,
+ * common_samplers.py:14-162, shallow_nets.py:87-108). */
+int qtx_rbm_metropolis(void* stream, int dtype, const void* packed, int32_t M, int32_t N, int kind,
+                       const int32_t* nbr, int32_t width, int32_t hopping, const uint32_t* words,
+                       int32_t nsteps, int64_t ns, double reweight, double tie_tol, int8_t* s,
+                       double* logabs, int32_t* n_accept, int32_t* n_near);
+
+/* Local energy E_loc(s) = diag + sum_k c_k psi(s_k')/psi(s) with local updates
+ * of h (operator.py:510-562 with Variational.ref_forward, variational.py:387-422).
+ * nconn (nullable) receives the number of valid candidates per sample. */
+int qtx_rbm_eloc(void* stream, int dtype, const void* packed, int32_t M, int32_t N, const qtx_ham* H,
+                 const int8_t* s, int64_t ns, double* eloc, int32_t* nconn);
+
+/* Per-sample log-derivatives O (ns, M*N + M) in o_dtype: [tanh(h) (x) s, tanh(h)]
+ * (variational.py:424-511 for SingleDense). */
+int qtx_rbm_jacobian(void* stream, int dtype, const void* packed, int32_t M, int32_t N, const int8_t* s,
+                     int64_t ns, int o_dtype, void* O);
+
+/* ---------------------------------------------------------------- ResConv
+ * ResConv(nblocks, channels, kernel 3) on an L0 x L1 periodic grid
+ * (model/conv_nets.py:26-183, nn/conv.py:13-68).  channels 16 (L0 * L1 <= 512)
+ * or 32 (L0 * L1 <= 400), nblocks <= 64.  gelu_exact = 0 uses the tanh form. */
+typedef struct qtx_resconv_desc {
+  int32_t nblocks;
+  int32_t channels;
+  int32_t L0;
+  int32_t L1;
+  int32_t precision; /* QTX_CONV_* */
+  int32_t gelu_exact;
+} qtx_resconv_desc;
+
+int64_t qtx_resconv_nparams(const qtx_resconv_desc* d);
+size_t qtx_resconv_packed_bytes(const qtx_resconv_desc* d);
+/* theta: float32 ravel_pytree vector (qtx_resconv_nparams entries). */
+int qtx_resconv_pack(void* stream, const qtx_resconv_desc* d, const float* theta, void* packed);
+
+/* log psi for a batch (Variational.__call__ for ResConv). */
+int qtx_resconv_logpsi(void* stream, const qtx_resconv_desc* d, const void* packed, const int8_t* s,
+                       int64_t ns, double* logabs);
+
+/* Metropolis steps with one full forward per proposal (ResConv is not a
+ * RefModel: variational.py:383-384).  logabs: in = current, out = final. */
+int qtx_resconv_metropolis(void* stream, const qtx_resconv_desc* d, const void* packed, int kind,
+                           const int32_t* nbr, int32_t width, int32_t hopping, const uint32_t* words,
+                           int32_t nsteps, int64_t ns, double reweight, double tie_tol, int8_t* s,
+                           double* logabs, int32_t* n_accept, int32_t* n_near);
+
+/* Local energy: one full forward per valid connected configuration, never
+ * materialised.  logabs = log psi(s) of the samples. */
+int qtx_resconv_eloc(void* stream, const qtx_resconv_desc* d, const void* packed, const qtx_ham* H,
+                     const int8_t* s, const double* logabs, int64_t ns, double* eloc, int32_t* nconn);
+
+/* Per-sample log-derivatives O (ns, nparams) float32 via a per-sample
+ * backward pass.  ws: qtx_resconv_jacobian_ws_bytes(d, ns) bytes. */
+size_t qtx_resconv_jacobian_ws_bytes(const qtx_resconv_desc* d, int64_t ns);
+int qtx_resconv_jacobian(void* stream, const qtx_resconv_desc* d, const void* packed, const int8_t* s,
+                         int64_t ns, float* O, void* ws, size_t ws_bytes);
+
+/* ---------------------------------------------------------------- SR / MinSR
+ * Column means of O (ns, np) in o_dtype, accumulated in float64
+ * (the Omean of optimizer/sr.py:79-88). */
+int qtx_colmean(void* stream, int o_dtype, const void* O, int64_t ns, int64_t np, double* mean);
+
+/* Ebar = (E - mean E) sqrt(r/ns); out2 = {energy, VarE} (sr.py:180-195).
+ * r nullable (= 1). */
+int qtx_ebar(void* stream, const double* eloc, const double* r, int64_t ns, double* ebar, double* out2);
+
+/* step = solver(Obar, Ebar) with Obar = (O - mean) sqrt(r/ns) formed on the
+ * fly.  ns < np: MinSR, T = Obar Obar^T (ns x ns); else SR, S = Obar^T Obar.
+ * mode CHOL_SHIFT: shift = rshift * trace + ashift (solver.py:50-91);
+ * mode EIGH_PINV : soft inverse 1/(v (1 + ((rtol max|v| + atol)/|v|)^6))
+ *                  (solver.py:94-201, tol_snr = 0).
+ * gram: QTX_GRAM_* (the ns >= np branch always uses FP64 products).
+ * ws: qtx_minsr_ws_bytes(...) bytes of device memory. */
+size_t qtx_minsr_ws_bytes(int64_t ns, int64_t np, int mode, int gram);
+int qtx_minsr_solve(void* stream, int o_dtype, const void* O, const double* mean, const double* r,
+                    const double* ebar, int64_t ns, int64_t np, int mode, int gram, double rshift,
+                    double ashift, double rtol, double atol, double* step, void* ws, size_t ws_bytes);
+
+/* theta <- theta - lr * step in dtype, skipped entirely when any step entry is
+ * non-finite (state/variational.py:558-579).  *applied (device int32) = 1/0. */
+int qtx_apply_update(void* stream, int dtype, void* theta, const double* step, double lr, int64_t np,
+                     int32_t* applied);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QTX_HIP_H */

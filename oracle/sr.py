@@ -1,0 +1,110 @@
+"""SR / MinSR step in float64 (oracle).
+
+Restates ``quantax/optimizer/sr.py:74-195`` (Obar, Ebar, energy, VarE, solve)
+and ``quantax/optimizer/solver.py:12-201`` (shifted Cholesky and soft
+pseudo-inverse solvers, automatic SR/MinSR choice), plus the parameter update
+``quantax/state/variational.py:558-579`` (skip on non-finite, theta -= step).
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.linalg as sla
+
+
+def get_Ebar(eloc, r=None):
+    """``SR.get_Ebar`` (sr.py:180-195): returns (Ebar, energy, VarE)."""
+    eloc = np.asarray(eloc, np.float64)
+    Ns = eloc.size
+    r = np.ones(Ns) if r is None else np.asarray(r, np.float64)
+    Emean = np.mean(eloc * r)
+    VarE = np.mean(np.abs(eloc - Emean) ** 2 * r)
+    Ebar = (eloc - np.mean(eloc)) * np.sqrt(r / Ns)
+    return Ebar, Emean, VarE
+
+
+def get_Obar(O, r=None):
+    """``QNGD.get_Obar`` / ``_Omat_to_Obar`` (sr.py:74-88): returns (Obar, Omean)."""
+    O = np.asarray(O, np.float64)
+    Ns = O.shape[0]
+    r = np.ones(Ns) if r is None else np.asarray(r, np.float64)
+    Omean = np.mean(O * r[:, None], axis=0)
+    Obar = (O - np.mean(O, axis=0, keepdims=True)) * np.sqrt(r / Ns)[:, None]
+    return Obar, Omean
+
+
+def _rtol(rtol):
+    return 1e-12 if rtol is None else rtol  # _get_rtol for float64 (solver.py:12-21)
+
+
+def minnorm_shift_eig(A, b, rshift=None, ashift=1e-4):
+    """solver.py:50-62: T = A A^T + (rshift tr T + ashift) I; x = A^T T^-1 b."""
+    T = A @ A.T
+    shift = _rtol(rshift) * np.trace(T) + ashift
+    T = T + shift * np.eye(T.shape[0])
+    y = sla.cho_solve(sla.cho_factor(T, lower=True), b)
+    return A.T @ y
+
+
+def lstsq_shift_eig(A, b, rshift=None, ashift=1e-4):
+    """solver.py:65-77: S = A^T A + (rshift tr S + ashift) I; x = S^-1 A^T b."""
+    S = A.T @ A
+    F = A.T @ b
+    shift = _rtol(rshift) * np.trace(S) + ashift
+    S = S + shift * np.eye(S.shape[0])
+    return sla.cho_solve(sla.cho_factor(S, lower=True), F)
+
+
+def auto_shift_eig(A, b, rshift=None, ashift=1e-4):
+    if A.shape[0] < A.shape[1]:
+        return minnorm_shift_eig(A, b, rshift, ashift)
+    return lstsq_shift_eig(A, b, rshift, ashift)
+
+
+def eigs_inv(vals, rtol=None, atol=0.0):
+    """``_get_eigs_inv`` (solver.py:94-101)."""
+    va = np.abs(vals)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        f = 1 + ((_rtol(rtol) * np.max(va) + atol) / va) ** 6
+        inv = 1.0 / (vals * f)
+    return np.where(va > 0.0, inv, 0.0)
+
+
+def minnorm_pinv_eig(A, b, rtol=None, atol=0.0):
+    """solver.py:128-149 (tol_snr = 0): x = A^T U diag(eig_inv) U^T b."""
+    T = A @ A.T
+    vals, U = np.linalg.eigh(T)
+    inv = eigs_inv(vals, rtol, atol)
+    rho = U.T @ b
+    return A.T @ (U @ (inv * rho))
+
+
+def lstsq_pinv_eig(A, b, rtol=None, atol=0.0):
+    """solver.py:152-164 (tol_snr = 0): x = V diag(eig_inv) V^T A^T b."""
+    S = A.T @ A
+    vals, V = np.linalg.eigh(S)
+    inv = eigs_inv(vals, rtol, atol)
+    rho = V.T @ (A.T @ b)
+    return V @ (inv * rho)
+
+
+def auto_pinv_eig(A, b, rtol=None, atol=0.0):
+    """solver.py:167-201."""
+    if A.shape[0] < A.shape[1]:
+        return minnorm_pinv_eig(A, b, rtol, atol)
+    return lstsq_pinv_eig(A, b, rtol, atol)
+
+
+def sr_step(O, eloc, solver, r=None):
+    """``QNGD.get_step`` (sr.py:115-123) for real parameters, imag_time=True."""
+    Ebar, E, VarE = get_Ebar(eloc, r)
+    Obar, _ = get_Obar(O, r)
+    return solver(Obar, Ebar), E, VarE
+
+
+def update(theta, step, dtype=None):
+    """``Variational.update`` (variational.py:558-579): None when skipped."""
+    step = np.asarray(step)
+    if not np.all(np.isfinite(step)):
+        return None
+    dtype = np.asarray(theta).dtype if dtype is None else dtype
+    return (np.asarray(theta) + (-step).astype(dtype)).astype(dtype)

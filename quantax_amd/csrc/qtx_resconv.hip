@@ -1,0 +1,1050 @@
+// ResConv(nblocks, C, 3) on a periodic L0 x L1 grid: batched forward on MFMA
+// (FP32 16x16x4, or split-bf16 16x16x32 with FP32 accumulation), Metropolis
+// with one full forward per proposal, local energy over connected
+// configurations generated in LDS, and a per-sample VALU backward for O.
+//
+// Forward work unit ("item") = one 16-position tile of one configuration.
+// Convolution as an implicit GEMM per item: D[co, p] = sum_k W[co, k] X[k, p]
+// with k = (tap, ci); A = weights (rows = output channels, register
+// resident per layer), B = activations read from LDS at the tap-shifted
+// position.  The residual stream stays in registers in the MFMA output
+// layout (lane l: channel 16 ct + 4 (l / 16) + q, position 16 t + l % 16).
+//
+// LDS activations: per configuration, per position, C/4 chunks of 16 bytes,
+// XOR-swizzled so 16 consecutive positions hit distinct banks.
+//   FP32   : chunk j = channels 4j .. 4j+3 (float)
+//   BF16X3 : chunk j < C/8 = hi(channels 8j .. 8j+7), chunk C/8 + j = lo(...)
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "common.hpp"
+
+namespace qtx {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+
+constexpr int RC_WAVES = 4;
+constexpr int RC_THREADS = RC_WAVES * WAVE;
+constexpr int RC_MAXN = 1024;
+
+// ---------------------------------------------------------------- layout
+struct RcLayout {
+  int nb, C, L0, L1, N, mode, gelu_exact;
+  int64_t nparams;
+  int64_t off_l0w, off_l0b;  // first conv (Cin = 1): W (C, 9), b (C) in floats
+  int64_t off_frag;          // start of per-layer fragment blocks (bytes)
+  int64_t frag_bytes;        // bytes of one layer's fragments
+  int64_t bias_off;          // start of per-layer biases (floats, (2nb) x C)
+  int64_t total_bytes;
+};
+
+__host__ __device__ inline int64_t rc_nparams(int nb, int C) {
+  int64_t n = 0;
+  for (int i = 0; i < nb; ++i) {
+    int cin = (i == 0) ? 1 : C;
+    n += (int64_t)C * cin * 9 + C;                 // conv1 (+ bias)
+    n += (int64_t)C * C * 9 + ((i == nb - 1) ? 0 : C);  // conv2 (+ bias except last)
+  }
+  return n;
+}
+
+// offsets of layer l's weight/bias in the ravel_pytree vector
+__host__ __device__ inline void rc_theta_offsets(int nb, int C, int l, int64_t& w, int64_t& b) {
+  int64_t off = 0;
+  for (int ll = 0; ll < 2 * nb; ++ll) {
+    int i = ll / 2;
+    int cin = (ll == 0) ? 1 : C;
+    bool hb = !((ll % 2 == 1) && (i == nb - 1));
+    if (ll == l) {
+      w = off;
+      b = hb ? off + (int64_t)C * cin * 9 : -1;
+      return;
+    }
+    off += (int64_t)C * cin * 9 + (hb ? C : 0);
+  }
+  w = b = -1;
+}
+
+template <int C, int MODE>
+struct RcK {
+  static constexpr int CT = C / 16;                                   // output-channel tiles
+  static constexpr int KS = (MODE == QTX_CONV_FP32) ? (9 * C / 4) : (C == 16 ? 5 : 9);  // K steps
+  static constexpr int CH = C / 4;                                    // 16-byte chunks per position
+  static constexpr int FRAG_WORDS = (MODE == QTX_CONV_FP32) ? 1 : 8;  // 4-byte words per lane per (ct, ks)
+};
+
+inline RcLayout rc_layout(const qtx_resconv_desc* d) {
+  RcLayout L;
+  L.nb = d->nblocks;
+  L.C = d->channels;
+  L.L0 = d->L0;
+  L.L1 = d->L1;
+  L.N = d->L0 * d->L1;
+  L.mode = d->precision;
+  L.gelu_exact = d->gelu_exact;
+  L.nparams = rc_nparams(L.nb, L.C);
+  int64_t off = L.nparams * 4;  // raw theta copy first
+  off = (off + 255) & ~(int64_t)255;
+  L.off_l0w = off / 4;
+  off += (int64_t)L.C * 9 * 4;
+  L.off_l0b = off / 4;
+  off += (int64_t)L.C * 4;
+  off = (off + 255) & ~(int64_t)255;
+  L.bias_off = off / 4;
+  off += (int64_t)2 * L.nb * L.C * 4;
+  off = (off + 255) & ~(int64_t)255;
+  L.off_frag = off;
+  int ct = L.C / 16;
+  int ks = (L.mode == QTX_CONV_FP32) ? 9 * L.C / 4 : (L.C == 16 ? 5 : 9);
+  int words = (L.mode == QTX_CONV_FP32) ? 1 : 8;
+  L.frag_bytes = (int64_t)ct * ks * WAVE * words * 4;
+  off += L.frag_bytes * 2 * L.nb;  // slot per layer index (slot 0 unused)
+  L.total_bytes = off;
+  return L;
+}
+
+// ---------------------------------------------------------------- packing
+// one thread per (layer, ct, ks, lane) fragment for layers 1 .. 2nb-1
+template <int C, int MODE>
+__global__ void rc_pack_frag_kernel(const float* __restrict__ theta, int nb, char* __restrict__ packed,
+                                    int64_t off_frag, int64_t frag_bytes) {
+  using K = RcK<C, MODE>;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t per_layer = (int64_t)K::CT * K::KS * WAVE;
+  int64_t total = per_layer * (2 * nb - 1);
+  if (idx >= total) return;
+  int l = 1 + (int)(idx / per_layer);
+  int64_t rem = idx % per_layer;
+  int ct = (int)(rem / (K::KS * WAVE));
+  int ks = (int)((rem / WAVE) % K::KS);
+  int lane = (int)(rem % WAVE);
+  int64_t w0, b0;
+  rc_theta_offsets(nb, C, l, w0, b0);
+  const int co = 16 * ct + lane % 16, g = lane / 16;
+  auto W = [&](int ci, int tap) -> float { return theta[w0 + ((int64_t)co * C + ci) * 9 + tap]; };
+  char* dst = packed + off_frag + (int64_t)l * frag_bytes;
+  if constexpr (MODE == QTX_CONV_FP32) {
+    // ks = (tap, h, j'): ci = 16 h + 4 g + j'
+    int tap = ks / (C / 4), rr = ks % (C / 4), h = rr / 4, jp = rr % 4;
+    float v = W(16 * h + 4 * g + jp, tap);
+    reinterpret_cast<float*>(dst)[((int64_t)ct * K::KS + ks) * WAVE + lane] = v;
+  } else {
+    int tap, c0;
+    if (C == 16) {
+      tap = 2 * ks + (g >> 1);
+      c0 = 8 * (g & 1);
+    } else {
+      tap = ks;
+      c0 = 8 * g;
+    }
+    __bf16* o = reinterpret_cast<__bf16*>(dst) + (((int64_t)ct * K::KS + ks) * WAVE + lane) * 16;
+    for (int q = 0; q < 8; ++q) {
+      float v = (tap < 9) ? W(c0 + q, tap) : 0.0f;
+      __bf16 hi = (__bf16)v;
+      __bf16 lo = (__bf16)(v - (float)hi);
+      o[q] = hi;
+      o[8 + q] = lo;
+    }
+  }
+}
+
+__global__ void rc_pack_misc_kernel(const float* __restrict__ theta, int nb, int C, int64_t nparams,
+                                    float* __restrict__ packed, int64_t off_l0w, int64_t off_l0b, int64_t bias_off) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < nparams) packed[idx] = theta[idx];
+  int64_t j = idx - nparams;
+  if (j >= 0 && j < (int64_t)C * 9) packed[off_l0w + j] = theta[j];  // layer 0 weights start at 0
+  j -= (int64_t)C * 9;
+  if (j >= 0 && j < C) packed[off_l0b + j] = theta[(int64_t)C * 9 + j];
+  j -= C;
+  if (j >= 0 && j < (int64_t)2 * nb * C) {
+    int l = (int)(j / C), co = (int)(j % C);
+    int64_t w0, b0;
+    rc_theta_offsets(nb, C, l, w0, b0);
+    packed[bias_off + j] = (b0 >= 0) ? theta[b0 + co] : 0.0f;
+  }
+}
+
+// ---------------------------------------------------------------- math
+__device__ __forceinline__ float gelu_f(float x, int exact) {
+  if (exact) return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f));
+  float u = 0.79788456080286536f * (x + 0.044715f * x * x * x);
+  float t = 1.0f - 2.0f / (1.0f + __expf(2.0f * u));
+  return 0.5f * x * (1.0f + t);
+}
+
+__device__ __forceinline__ float gelu_grad_f(float x, int exact) {
+  if (exact) return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * __expf(-0.5f * x * x) * 0.39894228040143268f;
+  float u = 0.79788456080286536f * (x + 0.044715f * x * x * x);
+  float t = 1.0f - 2.0f / (1.0f + __expf(2.0f * u));
+  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * 0.79788456080286536f * (1.0f + 0.134145f * x * x);
+}
+
+// ---------------------------------------------------------------- forward core
+template <int C>
+__device__ __forceinline__ int swz_chunk(int p, int j) {
+  constexpr int CH = C / 4;
+  return j ^ ((p / (16 / CH)) & (CH - 1));
+}
+
+struct RcGeom {
+  int L0, L1, N, NT;
+};
+
+__device__ __forceinline__ int tap_pos(const RcGeom& g, int y, int x, int tap) {
+  int dy = tap / 3 - 1, dx = tap % 3 - 1;
+  int ys = y + dy, xs = x + dx;
+  ys = ys < 0 ? ys + g.L0 : (ys >= g.L0 ? ys - g.L0 : ys);
+  xs = xs < 0 ? xs + g.L1 : (xs >= g.L1 ? xs - g.L1 : xs);
+  return ys * g.L1 + xs;
+}
+
+// Write 4 consecutive channels (c4 .. c4+3, c4 % 4 == 0) of position p.
+template <int C, int MODE>
+__device__ __forceinline__ void act_store4(char* act_cfg, int p, int c4, v4f v) {
+  constexpr int CH = C / 4;
+  if constexpr (MODE == QTX_CONV_FP32) {
+    int j = c4 / 4;
+    *reinterpret_cast<v4f*>(act_cfg + ((int64_t)p * CH + swz_chunk<C>(p, j)) * 16) = v;
+  } else {
+    v4bf hi, lo;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      __bf16 h = (__bf16)v[q];
+      hi[q] = h;
+      lo[q] = (__bf16)(v[q] - (float)h);
+    }
+    int j = c4 / 8, half = (c4 % 8) / 4;
+    char* base = act_cfg + (int64_t)p * CH * 16;
+    *reinterpret_cast<v4bf*>(base + swz_chunk<C>(p, j) * 16 + half * 8) = hi;
+    *reinterpret_cast<v4bf*>(base + swz_chunk<C>(p, C / 8 + j) * 16 + half * 8) = lo;
+  }
+}
+
+// Accumulate one conv layer (C -> C) for one item into acc[CT].
+template <int C, int MODE>
+__device__ __forceinline__ void conv_item(const char* act_cfg, const RcGeom& g, int p, int y, int x, int lane,
+                                          const float* wf, const v8bf* wh, const v8bf* wl, v4f* acc) {
+  using K = RcK<C, MODE>;
+  constexpr int CH = K::CH;
+  const int gq = lane / 16;
+  if constexpr (MODE == QTX_CONV_FP32) {
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      int ps = tap_pos(g, y, x, tap);
+      const char* base = act_cfg + (int64_t)ps * CH * 16;
+#pragma unroll
+      for (int h = 0; h < C / 16; ++h) {
+        int j = 4 * h + gq;
+        v4f bv = *reinterpret_cast<const v4f*>(base + swz_chunk<C>(ps, j) * 16);
+#pragma unroll
+        for (int jp = 0; jp < 4; ++jp) {
+          int ks = tap * (C / 4) + 4 * h + jp;
+#pragma unroll
+          for (int ct = 0; ct < K::CT; ++ct)
+            acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(wf[ct * K::KS + ks], bv[jp], acc[ct], 0, 0, 0);
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int ks = 0; ks < K::KS; ++ks) {
+      int tap, c0;
+      if (C == 16) {
+        tap = 2 * ks + (gq >> 1);
+        c0 = 8 * (gq & 1);
+      } else {
+        tap = ks;
+        c0 = 8 * gq;
+      }
+      int ps = (tap < 9) ? tap_pos(g, y, x, tap) : p;
+      const char* base = act_cfg + (int64_t)ps * CH * 16;
+      int jh = c0 / 8;
+      v8bf bh = *reinterpret_cast<const v8bf*>(base + swz_chunk<C>(ps, jh) * 16);
+      v8bf bl = *reinterpret_cast<const v8bf*>(base + swz_chunk<C>(ps, C / 8 + jh) * 16);
+#pragma unroll
+      for (int ct = 0; ct < K::CT; ++ct) {
+        acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[ct * K::KS + ks], bh, acc[ct], 0, 0, 0);
+        acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[ct * K::KS + ks], bl, acc[ct], 0, 0, 0);
+        acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[ct * K::KS + ks], bh, acc[ct], 0, 0, 0);
+      }
+    }
+  }
+}
+
+// Forward of B configurations whose spins are in sp[b * N + i] (LDS int8).
+// Result: lp_out[b] (LDS double).  All threads of the block must call it.
+// act: two LDS activation buffers of B * N * C * 4 bytes each (ping-pong: a
+// layer reads one and writes the other, so one barrier per layer suffices).
+// red: >= 2 * B * RC_WAVES floats.  Items of wave w: w + RC_WAVES * k, k < MAXT.
+template <int C, int MODE, int B, int MAXT>
+__device__ void rc_forward(const RcGeom& g, int nb, int gelu_exact, const char* __restrict__ packed,
+                           int64_t off_l0w, int64_t off_l0b, int64_t bias_off, int64_t off_frag, int64_t frag_bytes,
+                           const int8_t* sp, char* act, float* red, double* lp_out) {
+  using K = RcK<C, MODE>;
+  constexpr int CT = K::CT;
+  const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+  const int gq = lane / 16;
+  const int nitems = B * g.NT;
+  const int64_t act_cfg_bytes = (int64_t)g.N * C * 4;
+  const int64_t buf_bytes = act_cfg_bytes * B;
+  const float* pf = reinterpret_cast<const float*>(packed);
+
+  v4f res[MAXT][CT];
+
+  // ---- layer 0: conv (Cin = 1) of s / sqrt(2) on VALU, GELU, into buffer 0
+  {
+    const float* W0 = pf + off_l0w;
+    const float* b0 = pf + off_l0b;
+#pragma unroll
+    for (int k = 0; k < MAXT; ++k) {
+      int it = wv + RC_WAVES * k;
+      if (it < nitems) {
+        int b = it / g.NT, t = it % g.NT;
+        int p = 16 * t + lane % 16;
+        bool pv = p < g.N;
+        int pc = pv ? p : 0;
+        int y = pc / g.L1, x = pc % g.L1;
+        const int8_t* s = sp + b * g.N;
+        float sv[9];
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) sv[tap] = (float)s[tap_pos(g, y, x, tap)] * 0.70710678118654752f;
+        float spin = (float)s[pc];
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          v4f z;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            int co = 16 * ct + 4 * gq + q;
+            float a = b0[co];
+#pragma unroll
+            for (int tap = 0; tap < 9; ++tap) a += W0[co * 9 + tap] * sv[tap];
+            z[q] = gelu_f(a, gelu_exact);
+          }
+          res[k][ct] = v4f{spin, spin, spin, spin};
+          if (pv) act_store4<C, MODE>(act + b * act_cfg_bytes, p, 16 * ct + 4 * gq, z);
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- layers 1 .. 2nb-1 on MFMA
+  int cur = 0;
+  for (int l = 1; l < 2 * nb; ++l) {
+    float wf[(MODE == QTX_CONV_FP32) ? CT * K::KS : 1];
+    v8bf wh[(MODE == QTX_CONV_FP32) ? 1 : CT * K::KS];
+    v8bf wl[(MODE == QTX_CONV_FP32) ? 1 : CT * K::KS];
+    const char* fr = packed + off_frag + (int64_t)l * frag_bytes;
+    if constexpr (MODE == QTX_CONV_FP32) {
+#pragma unroll
+      for (int i = 0; i < CT * K::KS; ++i) wf[i] = reinterpret_cast<const float*>(fr)[i * WAVE + lane];
+    } else {
+#pragma unroll
+      for (int i = 0; i < CT * K::KS; ++i) {
+        const v8bf* src = reinterpret_cast<const v8bf*>(fr) + ((int64_t)i * WAVE + lane) * 2;
+        wh[i] = src[0];
+        wl[i] = src[1];
+      }
+    }
+    const float* bias = pf + bias_off + (int64_t)l * C;
+    const char* in = act + cur * buf_bytes;
+    char* out = act + (cur ^ 1) * buf_bytes;
+    const bool is_conv2 = (l % 2) == 1;
+    const int blk = l / 2;
+    const bool last = (l == 2 * nb - 1);
+    const float scale = last ? rsqrtf((float)(nb + 1)) : (is_conv2 ? rsqrtf((float)(blk + 2)) : 1.0f);
+    float m_b[B], s_b[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      m_b[b] = -INFINITY;
+      s_b[b] = 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < MAXT; ++k) {
+      int it = wv + RC_WAVES * k;
+      if (it < nitems) {
+        int b = it / g.NT, t = it % g.NT;
+        int p = 16 * t + lane % 16;
+        int pc = p < g.N ? p : 0;
+        int y = pc / g.L1, x = pc % g.L1;
+        v4f acc[CT];
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          int cb = 16 * ct + 4 * gq;
+          acc[ct] = v4f{bias[cb], bias[cb + 1], bias[cb + 2], bias[cb + 3]};
+        }
+        conv_item<C, MODE>(in + b * act_cfg_bytes, g, pc, y, x, lane, wf, wh, wl, acc);
+        if (!last) {
+#pragma unroll
+          for (int ct = 0; ct < CT; ++ct) {
+            v4f o;
+            if (is_conv2) {
+              v4f xr = acc[ct] + res[k][ct];
+              res[k][ct] = xr;
+#pragma unroll
+              for (int q = 0; q < 4; ++q) o[q] = gelu_f(xr[q] * scale, gelu_exact);
+            } else {
+#pragma unroll
+              for (int q = 0; q < 4; ++q) o[q] = gelu_f(acc[ct][q], gelu_exact);
+            }
+            if (p < g.N) act_store4<C, MODE>(out + b * act_cfg_bytes, p, 16 * ct + 4 * gq, o);
+          }
+        } else {
+          // head: h = (z2 + r) / sqrt(nb + 1); running logsumexp per configuration
+          float m = -INFINITY;
+          float hv[CT * 4];
+#pragma unroll
+          for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              float h = (acc[ct][q] + res[k][ct][q]) * scale;
+              hv[ct * 4 + q] = h;
+              if (p < g.N) m = fmaxf(m, h);
+            }
+          m = wave_max(m);
+          float sacc = 0.0f;
+          if (p < g.N) {
+#pragma unroll
+            for (int i = 0; i < CT * 4; ++i) sacc += __expf(hv[i] - m);
+          }
+          sacc = wave_sum(sacc);
+#pragma unroll
+          for (int bb = 0; bb < B; ++bb) {
+            if (bb == b) {
+              float mm = fmaxf(m_b[bb], m);
+              s_b[bb] = s_b[bb] * __expf(m_b[bb] - mm) + sacc * __expf(m - mm);
+              m_b[bb] = mm;
+            }
+          }
+        }
+      }
+    }
+    if (!last) {
+      __syncthreads();
+      cur ^= 1;
+    } else {
+      if (lane == 0) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+          red[(b * RC_WAVES + wv) * 2] = m_b[b];
+          red[(b * RC_WAVES + wv) * 2 + 1] = s_b[b];
+        }
+      }
+      __syncthreads();
+      if (threadIdx.x < B) {
+        int b = threadIdx.x;
+        float m = -INFINITY;
+        for (int w = 0; w < RC_WAVES; ++w) m = fmaxf(m, red[(b * RC_WAVES + w) * 2]);
+        double ssum = 0.0;
+        for (int w = 0; w < RC_WAVES; ++w) {
+          float mw = red[(b * RC_WAVES + w) * 2];
+          if (mw > -INFINITY) ssum += (double)red[(b * RC_WAVES + w) * 2 + 1] * exp((double)mw - (double)m);
+        }
+        lp_out[b] = (double)m + log(ssum) - log((double)C * (double)g.N);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+}  // namespace qtx
+
+namespace qtx {
+
+// ---------------------------------------------------------------- kernels
+struct RcArgs {
+  RcGeom g;
+  int nb, gelu_exact;
+  const char* packed;
+  int64_t off_l0w, off_l0b, bias_off, off_frag, frag_bytes;
+};
+
+#define RC_FWD(SP, LP) \
+  rc_forward<C, MODE, B, MAXT>(a.g, a.nb, a.gelu_exact, a.packed, a.off_l0w, a.off_l0b, a.bias_off, a.off_frag, \
+                               a.frag_bytes, SP, act, red, LP)
+
+template <int C, int MODE, int B, int NMAX, int MAXT>
+__global__ __launch_bounds__(RC_THREADS) void rc_logpsi_kernel(RcArgs a, const int8_t* __restrict__ s, int64_t ns,
+                                                               double* __restrict__ logabs) {
+  __shared__ __attribute__((aligned(16))) char act[2 * B * NMAX * C * 4];
+  __shared__ int8_t sp[B * NMAX];
+  __shared__ float red[2 * B * RC_WAVES];
+  __shared__ double lp[B];
+  const int N = a.g.N;
+  int64_t c0 = (int64_t)blockIdx.x * B;
+  for (int i = threadIdx.x; i < B * N; i += RC_THREADS) {
+    int b = i / N;
+    sp[i] = (c0 + b < ns) ? s[(c0 + b) * N + (i % N)] : (int8_t)1;
+  }
+  __syncthreads();
+  RC_FWD(sp, lp);
+  if (threadIdx.x < B && c0 + threadIdx.x < ns) logabs[c0 + threadIdx.x] = lp[threadIdx.x];
+}
+
+template <int C, int MODE, int B, int NMAX, int MAXT>
+__global__ __launch_bounds__(RC_THREADS) void rc_metropolis_kernel(
+    RcArgs a, int kind, const int32_t* __restrict__ nbr, int width, int hopping, const uint32_t* __restrict__ words,
+    int nsteps, int64_t ns, double reweight, double tie_tol, int8_t* __restrict__ s, double* __restrict__ logabs,
+    int32_t* __restrict__ n_accept, int32_t* __restrict__ n_near) {
+  __shared__ __attribute__((aligned(16))) char act[2 * B * NMAX * C * 4];
+  __shared__ int8_t cur[B * NMAX];
+  __shared__ int8_t prop[B * NMAX];
+  __shared__ float red[2 * B * RC_WAVES];
+  __shared__ double lp_new[B];
+  __shared__ double lp_cur[B];
+  __shared__ int mv_a[B], mv_b[B], mv_upd[B], nhop[B], nacc[B], nnear[B];
+  __shared__ double mv_thr[B];
+  const int N = a.g.N;
+  const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+  int64_t c0 = (int64_t)blockIdx.x * B;
+  for (int i = threadIdx.x; i < B * N; i += RC_THREADS) {
+    int b = i / N;
+    cur[i] = (c0 + b < ns) ? s[(c0 + b) * N + (i % N)] : (int8_t)1;
+  }
+  if (threadIdx.x < B) {
+    int b = threadIdx.x;
+    lp_cur[b] = (c0 + b < ns) ? logabs[c0 + b] : 0.0;
+    nacc[b] = 0;
+    nnear[b] = 0;
+  }
+  __syncthreads();
+  for (int b = wv; b < B; b += RC_WAVES) {
+    int cnt = 0;
+    for (int i = lane; i < N; i += WAVE) cnt += (cur[b * N + i] == hopping);
+    cnt = wave_sum(cnt);
+    if (lane == 0) nhop[b] = cnt;
+  }
+  __syncthreads();
+  for (int t = 0; t < nsteps; ++t) {
+    // proposals: one wave per chain
+    for (int b = wv; b < B; b += RC_WAVES) {
+      int64_t c = c0 + b;
+      int fa = -1, fb = -1;
+      bool upd = false;
+      double thr = 0.0;
+      if (c < ns) {
+        uint4 w = reinterpret_cast<const uint4*>(words)[(int64_t)t * ns + c];
+        propose_move(kind, cur + b * N, N, nbr, width, hopping, nhop[b], w.x, w.y, lane, fa, fb, upd);
+        thr = log1p(-words_to_uniform(w.z, w.w));
+      }
+      for (int i = lane; i < N; i += WAVE) {
+        int8_t v = cur[b * N + i];
+        if (upd && (i == fa || i == fb)) v = -v;
+        prop[b * N + i] = v;
+      }
+      if (lane == 0) {
+        mv_a[b] = fa;
+        mv_b[b] = fb;
+        mv_upd[b] = upd ? 1 : 0;
+        mv_thr[b] = thr;
+      }
+    }
+    __syncthreads();
+    RC_FWD(prop, lp_new);
+    // accept / reject (rc_forward ends with a barrier)
+    for (int b = wv; b < B; b += RC_WAVES) {
+      if (!mv_upd[b]) continue;
+      double d = reweight * (lp_new[b] - lp_cur[b]);
+      double thr = mv_thr[b];
+      bool acc = (d > thr) || (lp_cur[b] == -INFINITY);
+      if (acc) {
+        int fa = mv_a[b], fb = mv_b[b];
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) {
+          cur[b * N + fa] = -cur[b * N + fa];
+          if (fb >= 0) cur[b * N + fb] = -cur[b * N + fb];
+          lp_cur[b] = lp_new[b];
+          nacc[b] += 1;
+        }
+      }
+      if (lane == 0 && fabs(d - thr) < tie_tol) nnear[b] += 1;
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < B * N; i += RC_THREADS) {
+    int b = i / N;
+    if (c0 + b < ns) s[(c0 + b) * N + (i % N)] = cur[i];
+  }
+  if (threadIdx.x < B && c0 + threadIdx.x < ns) {
+    int b = threadIdx.x;
+    logabs[c0 + b] = lp_cur[b];
+    n_accept[c0 + b] += nacc[b];
+    n_near[c0 + b] += nnear[b];
+  }
+}
+
+constexpr int RC_MAXCAND = 4096;
+
+template <int C, int MODE, int B, int NMAX, int MAXT>
+__global__ __launch_bounds__(RC_THREADS) void rc_eloc_kernel(RcArgs a, qtx_ham H, const int8_t* __restrict__ s,
+                                                             const double* __restrict__ logabs, int64_t ns,
+                                                             double* __restrict__ eloc, int32_t* __restrict__ nconn) {
+  __shared__ __attribute__((aligned(16))) char act[2 * B * NMAX * C * 4];
+  __shared__ int8_t s0[NMAX];
+  __shared__ int8_t prop[B * NMAX];
+  __shared__ float red[2 * B * RC_WAVES];
+  __shared__ double lp[B];
+  __shared__ int16_t vlist[RC_MAXCAND];
+  __shared__ int nvalid_s;
+  __shared__ double diag_s;
+  const int N = a.g.N;
+  const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+  const int64_t n = blockIdx.x;
+  for (int i = threadIdx.x; i < N; i += RC_THREADS) s0[i] = s[n * N + i];
+  __syncthreads();
+  if (wv == 0) {
+    // diagonal part and the ordered list of valid candidates (ballot compaction)
+    double dg = diag_energy(H, s0, lane);
+    int pos = 0;
+    for (int base = 0; base < H.noff; base += WAVE) {
+      int k = base + lane;
+      bool v = (k < H.noff) && cand_valid(H, s0, k);
+      uint64_t m = __ballot(v);
+      uint64_t below = (lane == 0) ? 0ull : (m & ((~0ull) >> (64 - lane)));
+      if (v) vlist[pos + __popcll(below)] = (int16_t)k;
+      pos += __popcll(m);
+    }
+    if (lane == 0) {
+      nvalid_s = pos;
+      diag_s = dg;
+    }
+  }
+  __syncthreads();
+  const int nvalid = nvalid_s;
+  const double lp0 = logabs[n];
+  double e = 0.0;  // accumulated by thread 0 in candidate order
+  for (int start = 0; start < nvalid; start += B) {
+    for (int i = threadIdx.x; i < B * N; i += RC_THREADS) {
+      int b = i / N, j = i % N;
+      int8_t v = s0[j];
+      if (start + b < nvalid) {
+        int k = vlist[start + b];
+        if (j == H.off_a[k] || j == H.off_b[k]) v = -v;
+      }
+      prop[i] = v;
+    }
+    __syncthreads();
+    RC_FWD(prop, lp);
+    if (threadIdx.x == 0) {
+      for (int b = 0; b < B && start + b < nvalid; ++b) e += H.off_c[vlist[start + b]] * exp(lp[b] - lp0);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    eloc[n] = diag_s + e;
+    if (nconn) nconn[n] = nvalid;
+  }
+}
+#undef RC_FWD
+
+
+// ---------------------------------------------------------------- Jacobian
+// One workgroup per sample.  Forward on VALU records, per block i, x_i (block
+// input), y_i (conv1 input), z1_i (conv1 output) and a1_i (conv2 input) in
+// channel-major [c][p] float layout; the backward then produces the O row
+// (ravel_pytree order: per layer weight (co, ci, tap), bias (co)).
+constexpr int JAC_THREADS = 256;
+
+template <int C>
+__device__ __forceinline__ void jac_conv_fwd(const float* __restrict__ in, int cin, const float* __restrict__ W,
+                                             const float* __restrict__ bias, const RcGeom& g, int p, float* out) {
+  int y = p / g.L1, x = p % g.L1;
+#pragma unroll
+  for (int co = 0; co < C; ++co) out[co] = bias ? bias[co] : 0.0f;
+  for (int ci = 0; ci < cin; ++ci)
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      float v = in[ci * g.N + tap_pos(g, y, x, tap)];
+#pragma unroll
+      for (int co = 0; co < C; ++co) out[co] += W[(co * cin + ci) * 9 + tap] * v;
+    }
+}
+
+// d in[ci][q] = sum_{co, tap} W[co][ci][tap] dz[co][q - shift(tap)]
+template <int C>
+__device__ __forceinline__ void jac_conv_bwd_in(const float* __restrict__ dz, int cin, const float* __restrict__ W,
+                                                const RcGeom& g, int q, float* din) {
+  int y = q / g.L1, x = q % g.L1;
+  for (int ci = 0; ci < cin; ++ci) din[ci] = 0.0f;
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap) {
+    int ps = tap_pos(g, y, x, 8 - tap);  // position p with shift_tap(p) = q
+    for (int co = 0; co < C; ++co) {
+      float d = dz[co * g.N + ps];
+      for (int ci = 0; ci < cin; ++ci) din[ci] += W[(co * cin + ci) * 9 + tap] * d;
+    }
+  }
+}
+
+// grads of W (co, ci, tap) and bias for one layer, written to the O row
+template <int C>
+__device__ __forceinline__ void jac_conv_grad_w(const float* __restrict__ dz, const float* __restrict__ in, int cin,
+                                                const RcGeom& g, float* __restrict__ Orow, int64_t w0, int64_t b0) {
+  for (int pair = threadIdx.x; pair < C * cin; pair += JAC_THREADS) {
+    int co = pair / cin, ci = pair % cin;
+    float acc[9];
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) acc[tap] = 0.0f;
+    for (int p = 0; p < g.N; ++p) {
+      float d = dz[co * g.N + p];
+      int y = p / g.L1, x = p % g.L1;
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap) acc[tap] += d * in[ci * g.N + tap_pos(g, y, x, tap)];
+    }
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) Orow[w0 + (int64_t)pair * 9 + tap] = acc[tap];
+  }
+  if (b0 >= 0) {
+    for (int co = threadIdx.x; co < C; co += JAC_THREADS) {
+      float acc = 0.0f;
+      for (int p = 0; p < g.N; ++p) acc += dz[co * g.N + p];
+      Orow[b0 + co] = acc;
+    }
+  }
+}
+
+template <int C, int NMAX>
+__global__ __launch_bounds__(JAC_THREADS) void rc_jacobian_kernel(RcGeom g, int nb, int gelu_exact,
+                                                                  const float* __restrict__ theta,
+                                                                  const int8_t* __restrict__ s, int64_t s0,
+                                                                  int64_t ns_chunk, int64_t np,
+                                                                  float* __restrict__ O, float* __restrict__ ws) {
+  __shared__ float bufX[C * NMAX];
+  __shared__ float bufA[C * NMAX];
+  __shared__ float bufB[C * NMAX];
+  __shared__ float red[2 * JAC_THREADS / WAVE];
+  const int64_t nl = blockIdx.x;
+  if (nl >= ns_chunk) return;
+  const int64_t n = s0 + nl;
+  const int N = g.N;
+  const int64_t blk_f = (int64_t)4 * C * N;
+  float* rec = ws + nl * blk_f * nb;
+  float* Orow = O + n * np;
+  const float isq2 = 0.70710678118654752f;
+
+  // ---- forward with record
+  for (int p = threadIdx.x; p < N; p += JAC_THREADS) bufX[p] = (float)s[n * N + p];
+  __syncthreads();
+  for (int i = 0; i < nb; ++i) {
+    const int cin = (i == 0) ? 1 : C;
+    float* rx = rec + i * blk_f;
+    float* ry = rx + C * N;
+    float* rz = ry + C * N;
+    float* ra = rz + C * N;
+    const float sc = rsqrtf((float)(i + 1));
+    for (int e = threadIdx.x; e < cin * N; e += JAC_THREADS) {
+      float xv = bufX[e];
+      rx[e] = xv;
+      float xin = xv * sc;
+      float yv = (i == 0) ? xin * isq2 : gelu_f(xin, gelu_exact);
+      bufA[e] = yv;
+      ry[e] = yv;
+    }
+    __syncthreads();
+    int64_t w1, b1, w2, b2;
+    rc_theta_offsets(nb, C, 2 * i, w1, b1);
+    rc_theta_offsets(nb, C, 2 * i + 1, w2, b2);
+    float out[C];
+    for (int p = threadIdx.x; p < N; p += JAC_THREADS) {
+      jac_conv_fwd<C>(bufA, cin, theta + w1, theta + b1, g, p, out);
+#pragma unroll
+      for (int co = 0; co < C; ++co) {
+        rz[co * N + p] = out[co];
+        bufB[co * N + p] = gelu_f(out[co], gelu_exact);
+        ra[co * N + p] = bufB[co * N + p];
+      }
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < N; p += JAC_THREADS) {
+      jac_conv_fwd<C>(bufB, C, theta + w2, b2 >= 0 ? theta + b2 : nullptr, g, p, out);
+      float r0 = bufX[p];  // channel 0 of the residual (block 0: the spin)
+#pragma unroll
+      for (int co = 0; co < C; ++co) out[co] += (cin == 1) ? r0 : bufX[co * N + p];
+#pragma unroll
+      for (int co = 0; co < C; ++co) bufA[co * N + p] = out[co];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < C * N; e += JAC_THREADS) bufX[e] = bufA[e];
+    __syncthreads();
+  }
+  // ---- head: dX = softmax(h) / sqrt(nb + 1), h = x / sqrt(nb + 1)
+  const float hs = rsqrtf((float)(nb + 1));
+  {
+    const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+    float m = -INFINITY;
+    for (int e = threadIdx.x; e < C * N; e += JAC_THREADS) m = fmaxf(m, bufX[e] * hs);
+    m = wave_max(m);
+    if (lane == 0) red[wv] = m;
+    __syncthreads();
+    m = -INFINITY;
+    for (int w = 0; w < JAC_THREADS / WAVE; ++w) m = fmaxf(m, red[w]);
+    float ssum = 0.0f;
+    for (int e = threadIdx.x; e < C * N; e += JAC_THREADS) ssum += expf(bufX[e] * hs - m);
+    ssum = wave_sum(ssum);
+    __syncthreads();
+    if (lane == 0) red[wv] = ssum;
+    __syncthreads();
+    ssum = 0.0f;
+    for (int w = 0; w < JAC_THREADS / WAVE; ++w) ssum += red[w];
+    float inv = 1.0f / ssum;
+    for (int e = threadIdx.x; e < C * N; e += JAC_THREADS) bufX[e] = expf(bufX[e] * hs - m) * inv * hs;
+    __syncthreads();
+  }
+  // ---- backward
+  for (int i = nb - 1; i >= 0; --i) {
+    const int cin = (i == 0) ? 1 : C;
+    float* rx = rec + i * blk_f;
+    float* ry = rx + C * N;
+    float* rz = ry + C * N;
+    float* ra = rz + C * N;
+    int64_t w1, b1, w2, b2;
+    rc_theta_offsets(nb, C, 2 * i, w1, b1);
+    rc_theta_offsets(nb, C, 2 * i + 1, w2, b2);
+    // conv2: dz2 = dX; input a1
+    for (int e = threadIdx.x; e < C * N; e += JAC_THREADS) bufA[e] = ra[e];
+    __syncthreads();
+    jac_conv_grad_w<C>(bufX, bufA, C, g, Orow, w2, b2);
+    __syncthreads();
+    // da1 -> dz1 = da1 * gelu'(z1), into bufB
+    {
+      float din[C];
+      for (int q = threadIdx.x; q < N; q += JAC_THREADS) {
+        jac_conv_bwd_in<C>(bufX, C, theta + w2, g, q, din);
+#pragma unroll
+        for (int ci = 0; ci < C; ++ci) bufB[ci * N + q] = din[ci] * gelu_grad_f(rz[ci * N + q], gelu_exact);
+      }
+    }
+    __syncthreads();
+    // conv1: input y_i
+    for (int e = threadIdx.x; e < cin * N; e += JAC_THREADS) bufA[e] = ry[e];
+    __syncthreads();
+    jac_conv_grad_w<C>(bufB, bufA, cin, g, Orow, w1, b1);
+    __syncthreads();
+    if (i > 0) {
+      float din[C];
+      const float sc = rsqrtf((float)(i + 1));
+      for (int q = threadIdx.x; q < N; q += JAC_THREADS) {
+        jac_conv_bwd_in<C>(bufB, C, theta + w1, g, q, din);
+#pragma unroll
+        for (int ci = 0; ci < C; ++ci) {
+          float xin = rx[ci * N + q] * sc;
+          bufA[ci * N + q] = din[ci] * gelu_grad_f(xin, gelu_exact) * sc + bufX[ci * N + q];
+        }
+      }
+      __syncthreads();
+      for (int e = threadIdx.x; e < C * N; e += JAC_THREADS) bufX[e] = bufA[e];
+      __syncthreads();
+    }
+  }
+}
+
+}  // namespace qtx
+
+using namespace qtx;
+
+// ================================================================ C ABI
+static int rc_check(const qtx_resconv_desc* d) {
+  QTX_REQUIRE(d != nullptr, "null qtx_resconv_desc");
+  QTX_REQUIRE(d->channels == 16 || d->channels == 32, "channels must be 16 or 32");
+  QTX_REQUIRE(d->nblocks >= 1 && d->nblocks <= 64, "nblocks out of range (1..64)");
+  QTX_REQUIRE(d->L0 >= 1 && d->L1 >= 1 && d->L0 * d->L1 <= RC_MAXN, "L0 * L1 out of range (1..1024)");
+  QTX_REQUIRE(d->precision == QTX_CONV_FP32 || d->precision == QTX_CONV_BF16X3, "unknown precision");
+  if (d->channels == 16) QTX_REQUIRE(d->L0 * d->L1 <= 512, "channels = 16 needs L0 * L1 <= 512");
+  if (d->channels == 32) QTX_REQUIRE(d->L0 * d->L1 <= 400, "channels = 32 needs L0 * L1 <= 400");
+  return QTX_OK;
+}
+
+static RcArgs rc_args(const RcLayout& L, const void* packed) {
+  RcArgs a;
+  a.g = RcGeom{L.L0, L.L1, L.N, (L.N + 15) / 16};
+  a.nb = L.nb;
+  a.gelu_exact = L.gelu_exact;
+  a.packed = (const char*)packed;
+  a.off_l0w = L.off_l0w;
+  a.off_l0b = L.off_l0b;
+  a.bias_off = L.bias_off;
+  a.off_frag = L.off_frag;
+  a.frag_bytes = L.frag_bytes;
+  return a;
+}
+
+// Kernel configurations: (C, B, NMAX, MAXT) with B * ceil(NMAX/16) <= 4 * MAXT
+// (C = 16: N <= 256 -> 2 configurations per workgroup, N <= 512 -> 1; C = 32: N <= 400 -> 1).
+#define RC_DISPATCH(KERNEL, NS_PER_BLOCK_OUT, GRID_FROM_B, ...)                                        \
+  do {                                                                                                 \
+    if (L.C == 16 && L.N <= 256) {                                                                     \
+      constexpr int B_ = 2;                                                                            \
+      NS_PER_BLOCK_OUT = B_;                                                                           \
+      dim3 grid_ = GRID_FROM_B;                                                                        \
+      if (L.mode == QTX_CONV_FP32)                                                                     \
+        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_FP32, 2, 256, 8>), grid_, dim3(RC_THREADS), 0, st, __VA_ARGS__); \
+      else                                                                                             \
+        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_BF16X3, 2, 256, 8>), grid_, dim3(RC_THREADS), 0, st, __VA_ARGS__); \
+    } else if (L.C == 16) {                                                                            \
+      constexpr int B_ = 1;                                                                            \
+      NS_PER_BLOCK_OUT = B_;                                                                           \
+      dim3 grid_ = GRID_FROM_B;                                                                        \
+      if (L.mode == QTX_CONV_FP32)                                                                     \
+        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_FP32, 1, 512, 8>), grid_, dim3(RC_THREADS), 0, st, __VA_ARGS__); \
+      else                                                                                             \
+        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_BF16X3, 1, 512, 8>), grid_, dim3(RC_THREADS), 0, st, __VA_ARGS__); \
+    } else {                                                                                           \
+      constexpr int B_ = 1;                                                                            \
+      NS_PER_BLOCK_OUT = B_;                                                                           \
+      dim3 grid_ = GRID_FROM_B;                                                                        \
+      if (L.mode == QTX_CONV_FP32)                                                                     \
+        hipLaunchKernelGGL((KERNEL<32, QTX_CONV_FP32, 1, 400, 7>), grid_, dim3(RC_THREADS), 0, st, __VA_ARGS__); \
+      else                                                                                             \
+        hipLaunchKernelGGL((KERNEL<32, QTX_CONV_BF16X3, 1, 400, 7>), grid_, dim3(RC_THREADS), 0, st, __VA_ARGS__); \
+    }                                                                                                  \
+  } while (0)
+
+extern "C" {
+
+int64_t qtx_resconv_nparams(const qtx_resconv_desc* d) {
+  if (!d || d->nblocks < 1 || d->channels < 1) return -1;
+  return rc_nparams(d->nblocks, d->channels);
+}
+
+size_t qtx_resconv_packed_bytes(const qtx_resconv_desc* d) {
+  if (rc_check(d) != QTX_OK) return 0;
+  return (size_t)rc_layout(d).total_bytes;
+}
+
+int qtx_resconv_pack(void* stream, const qtx_resconv_desc* d, const float* theta, void* packed) {
+  int rc = rc_check(d);
+  if (rc) return rc;
+  QTX_REQUIRE(theta && packed, "null pointer");
+  RcLayout L = rc_layout(d);
+  hipStream_t st = as_stream(stream);
+  int64_t misc = L.nparams + (int64_t)L.C * 10 + (int64_t)2 * L.nb * L.C;
+  hipLaunchKernelGGL(rc_pack_misc_kernel, dim3((unsigned)((misc + 255) / 256)), dim3(256), 0, st, theta, L.nb, L.C,
+                     L.nparams, (float*)packed, L.off_l0w, L.off_l0b, L.bias_off);
+  int ct = L.C / 16;
+  int ks = (L.mode == QTX_CONV_FP32) ? 9 * L.C / 4 : (L.C == 16 ? 5 : 9);
+  int64_t total = (int64_t)ct * ks * WAVE * (2 * L.nb - 1);
+  if (total > 0) {
+    dim3 g((unsigned)((total + 255) / 256)), b(256);
+    if (L.C == 16 && L.mode == QTX_CONV_FP32)
+      hipLaunchKernelGGL((rc_pack_frag_kernel<16, QTX_CONV_FP32>), g, b, 0, st, theta, L.nb, (char*)packed,
+                         L.off_frag, L.frag_bytes);
+    else if (L.C == 16)
+      hipLaunchKernelGGL((rc_pack_frag_kernel<16, QTX_CONV_BF16X3>), g, b, 0, st, theta, L.nb, (char*)packed,
+                         L.off_frag, L.frag_bytes);
+    else if (L.mode == QTX_CONV_FP32)
+      hipLaunchKernelGGL((rc_pack_frag_kernel<32, QTX_CONV_FP32>), g, b, 0, st, theta, L.nb, (char*)packed,
+                         L.off_frag, L.frag_bytes);
+    else
+      hipLaunchKernelGGL((rc_pack_frag_kernel<32, QTX_CONV_BF16X3>), g, b, 0, st, theta, L.nb, (char*)packed,
+                         L.off_frag, L.frag_bytes);
+  }
+  QTX_LAUNCH_CHECK();
+  return QTX_OK;
+}
+
+int qtx_resconv_logpsi(void* stream, const qtx_resconv_desc* d, const void* packed, const int8_t* s, int64_t ns,
+                       double* logabs) {
+  int rc = rc_check(d);
+  if (rc) return rc;
+  QTX_REQUIRE(packed && ns >= 0 && (ns == 0 || (s && logabs)), "bad arguments");
+  if (ns == 0) return QTX_OK;
+  RcLayout L = rc_layout(d);
+  RcArgs a = rc_args(L, packed);
+  hipStream_t st = as_stream(stream);
+  int per = 1;
+  RC_DISPATCH(rc_logpsi_kernel, per, dim3((unsigned)((ns + B_ - 1) / B_)), a, s, ns, logabs);
+  (void)per;
+  QTX_LAUNCH_CHECK();
+  return QTX_OK;
+}
+
+int qtx_resconv_metropolis(void* stream, const qtx_resconv_desc* d, const void* packed, int kind,
+                           const int32_t* nbr, int32_t width, int32_t hopping, const uint32_t* words,
+                           int32_t nsteps, int64_t ns, double reweight, double tie_tol, int8_t* s, double* logabs,
+                           int32_t* n_accept, int32_t* n_near) {
+  int rc = rc_check(d);
+  if (rc) return rc;
+  QTX_REQUIRE(kind == QTX_LOCALFLIP || kind == QTX_EXCHANGE, "kind must be QTX_LOCALFLIP or QTX_EXCHANGE");
+  QTX_REQUIRE(kind == QTX_LOCALFLIP || (nbr != nullptr && width > 0), "EXCHANGE needs a neighbour table");
+  QTX_REQUIRE(hopping == 1 || hopping == -1, "hopping must be +1 or -1");
+  QTX_REQUIRE(nsteps >= 0 && ns >= 0, "negative size");
+  QTX_REQUIRE(packed && (ns == 0 || (s && logabs && n_accept && n_near && (nsteps == 0 || words))),
+              "null pointer");
+  if (ns == 0) return QTX_OK;
+  RcLayout L = rc_layout(d);
+  RcArgs a = rc_args(L, packed);
+  hipStream_t st = as_stream(stream);
+  int per = 1;
+  RC_DISPATCH(rc_metropolis_kernel, per, dim3((unsigned)((ns + B_ - 1) / B_)), a, kind, nbr, width, hopping, words,
+              nsteps, ns, reweight, tie_tol, s, logabs, n_accept, n_near);
+  (void)per;
+  QTX_LAUNCH_CHECK();
+  return QTX_OK;
+}
+
+int qtx_resconv_eloc(void* stream, const qtx_resconv_desc* d, const void* packed, const qtx_ham* H,
+                     const int8_t* s, const double* logabs, int64_t ns, double* eloc, int32_t* nconn) {
+  int rc = rc_check(d);
+  if (rc) return rc;
+  QTX_REQUIRE(H != nullptr, "null qtx_ham");
+  QTX_REQUIRE(H->N == d->L0 * d->L1, "qtx_ham.N != L0 * L1");
+  QTX_REQUIRE(H->noff <= RC_MAXCAND, "too many off-diagonal candidates (max 4096)");
+  QTX_REQUIRE(H->noff == 0 || (H->off_a && H->off_b && H->off_ra && H->off_rb && H->off_c), "null table");
+  QTX_REQUIRE(H->ndiag == 0 || (H->diag_c && H->diag_a && H->diag_b && H->diag_kind), "null table");
+  QTX_REQUIRE(packed && ns >= 0 && (ns == 0 || (s && logabs && eloc)), "bad arguments");
+  if (ns == 0) return QTX_OK;
+  QTX_REQUIRE(ns <= 0x7fffffff, "ns too large");
+  RcLayout L = rc_layout(d);
+  RcArgs a = rc_args(L, packed);
+  hipStream_t st = as_stream(stream);
+  int per = 1;
+  RC_DISPATCH(rc_eloc_kernel, per, dim3((unsigned)ns), a, *H, s, logabs, ns, eloc, nconn);
+  (void)per;
+  QTX_LAUNCH_CHECK();
+  return QTX_OK;
+}
+
+static int64_t jac_chunk(int64_t ns) { return ns < 2048 ? ns : 2048; }
+
+size_t qtx_resconv_jacobian_ws_bytes(const qtx_resconv_desc* d, int64_t ns) {
+  if (rc_check(d) != QTX_OK || ns <= 0) return 0;
+  int64_t N = (int64_t)d->L0 * d->L1;
+  return (size_t)(jac_chunk(ns) * (int64_t)d->nblocks * 4 * d->channels * N * 4);
+}
+
+int qtx_resconv_jacobian(void* stream, const qtx_resconv_desc* d, const void* packed, const int8_t* s, int64_t ns,
+                         float* O, void* ws, size_t ws_bytes) {
+  int rc = rc_check(d);
+  if (rc) return rc;
+  QTX_REQUIRE(packed && ns >= 0 && (ns == 0 || (s && O)), "bad arguments");
+  if (ns == 0) return QTX_OK;
+  size_t need = qtx_resconv_jacobian_ws_bytes(d, ns);
+  if (ws == nullptr || ws_bytes < need)
+    return fail(QTX_EWORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+  RcLayout L = rc_layout(d);
+  RcGeom g{L.L0, L.L1, L.N, (L.N + 15) / 16};
+  hipStream_t st = as_stream(stream);
+  const float* theta = (const float*)packed;  // raw theta copy at offset 0
+  int64_t chunk = jac_chunk(ns);
+  for (int64_t s0 = 0; s0 < ns; s0 += chunk) {
+    int64_t nc = (ns - s0) < chunk ? (ns - s0) : chunk;
+    if (L.C == 16 && L.N <= 256)
+      hipLaunchKernelGGL((rc_jacobian_kernel<16, 256>), dim3((unsigned)nc), dim3(JAC_THREADS), 0, st, g, L.nb,
+                         L.gelu_exact, theta, s, s0, nc, L.nparams, O, (float*)ws);
+    else if (L.C == 16)
+      hipLaunchKernelGGL((rc_jacobian_kernel<16, 512>), dim3((unsigned)nc), dim3(JAC_THREADS), 0, st, g, L.nb,
+                         L.gelu_exact, theta, s, s0, nc, L.nparams, O, (float*)ws);
+    else
+      hipLaunchKernelGGL((rc_jacobian_kernel<32, 400>), dim3((unsigned)nc), dim3(JAC_THREADS), 0, st, g, L.nb,
+                         L.gelu_exact, theta, s, s0, nc, L.nparams, O, (float*)ws);
+    QTX_LAUNCH_CHECK();
+  }
+  return QTX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,592 @@
+// SR / MinSR: Omean, Ebar, the MFMA Gram of Obar, rocSOLVER Cholesky / eigh,
+// the update vector and the guarded parameter update.
+//
+// Obar = (O - mean) * sqrt(r / ns) is never materialised: the Gram kernel
+// centres each O tile in FP64 while staging it into LDS (then rounds to FP32
+// or splits it into bf16 hi/lo), and the update vector x = Obar^T y reads O
+// once more with FP64 accumulation.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <cmath>
+#include <mutex>
+#include <string>
+
+#include "common.hpp"
+
+namespace qtx {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// ---------------------------------------------------------------- helpers
+template <typename T>
+__device__ __forceinline__ double ld_as_double(const T* p, int64_t i) {
+  return (double)p[i];
+}
+
+__global__ void fill_f64_kernel(double* p, int64_t n, double v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// column sums over a chunk of rows, accumulated with FP64 atomics
+template <typename T>
+__global__ __launch_bounds__(256) void colsum_kernel(const T* __restrict__ O, int64_t ns, int64_t np,
+                                                     int64_t rows_per_chunk, double* __restrict__ out) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  int64_t r0 = (int64_t)blockIdx.y * rows_per_chunk;
+  int64_t r1 = r0 + rows_per_chunk < ns ? r0 + rows_per_chunk : ns;
+  double acc = 0.0;
+  for (int64_t r = r0; r < r1; ++r) acc += (double)O[r * np + k];
+  atomicAdd(out + k, acc);
+}
+
+__global__ void scale_f64_kernel(double* p, int64_t n, double a) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] *= a;
+}
+
+// single-block: energy, VarE and Ebar (sr.py:180-195)
+__global__ __launch_bounds__(1024) void ebar_kernel(const double* __restrict__ E, const double* __restrict__ r,
+                                                    int64_t ns, double* __restrict__ ebar, double* __restrict__ out2) {
+  __shared__ double red[3][16];
+  int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE, nw = blockDim.x / WAVE;
+  double s_er = 0, s_e = 0;
+  for (int64_t i = tid; i < ns; i += blockDim.x) {
+    double ri = r ? r[i] : 1.0;
+    s_er += E[i] * ri;
+    s_e += E[i];
+  }
+  s_er = wave_sum(s_er);
+  s_e = wave_sum(s_e);
+  if (lane == 0) {
+    red[0][wv] = s_er;
+    red[1][wv] = s_e;
+  }
+  __syncthreads();
+  double tot_er = 0, tot_e = 0;
+  for (int w = 0; w < nw; ++w) {
+    tot_er += red[0][w];
+    tot_e += red[1][w];
+  }
+  double Emean = tot_er / (double)ns, Eplain = tot_e / (double)ns;
+  double s_v = 0;
+  for (int64_t i = tid; i < ns; i += blockDim.x) {
+    double ri = r ? r[i] : 1.0;
+    double d = E[i] - Emean;
+    s_v += d * d * ri;
+    ebar[i] = (E[i] - Eplain) * sqrt(ri / (double)ns);
+  }
+  s_v = wave_sum(s_v);
+  __syncthreads();
+  if (lane == 0) red[2][wv] = s_v;
+  __syncthreads();
+  if (tid == 0) {
+    double tv = 0;
+    for (int w = 0; w < nw; ++w) tv += red[2][w];
+    out2[0] = Emean;
+    out2[1] = tv / (double)ns;
+  }
+}
+
+// ---------------------------------------------------------------- Gram T = A A^T
+// A = Obar (n x np) from O rows.  Workgroup: 256 threads = 2 x 2 waves, tile
+// 128 x 128 (each wave 64 x 64 = 4 x 4 MFMA 16x16 blocks); only tiles with
+// bi >= bj are computed and mirrored.  Output T is FP64 (n x n, symmetric).
+constexpr int GT = 128;
+
+template <int G>
+struct GramCfg;
+template <>
+struct GramCfg<QTX_GRAM_FP32> {
+  static constexpr int KC = 32;   // K elements staged per chunk
+  static constexpr int PAD = 1;   // LDS row padding (elements)
+};
+template <>
+struct GramCfg<QTX_GRAM_BF16X3> {
+  static constexpr int KC = 32;
+  static constexpr int PAD = 8;
+};
+template <>
+struct GramCfg<QTX_GRAM_FP64> {
+  static constexpr int KC = 16;
+  static constexpr int PAD = 1;
+};
+
+__device__ __forceinline__ void tile_of(int bid, int& bi, int& bj) {
+  // bid -> (bi, bj) with bj <= bi, row-major over the lower triangle
+  int b = (int)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
+  while ((b + 1) * (b + 2) / 2 <= bid) ++b;
+  while (b * (b + 1) / 2 > bid) --b;
+  bi = b;
+  bj = bid - b * (b + 1) / 2;
+}
+
+template <typename TI, int G>
+__global__ __launch_bounds__(256) void gram_nt_kernel(const TI* __restrict__ O, const double* __restrict__ mean,
+                                                      const double* __restrict__ r, int64_t n, int64_t np,
+                                                      double inv_ns, double* __restrict__ T) {
+  constexpr int KC = GramCfg<G>::KC;
+  constexpr int LD = KC + GramCfg<G>::PAD;
+  using ST = typename std::conditional<G == QTX_GRAM_FP64, double, float>::type;
+  // LDS: FP32/FP64 modes stage ST values; BF16X3 stages hi and lo planes
+  constexpr int PLANES = (G == QTX_GRAM_BF16X3) ? 2 : 1;
+  using LT = typename std::conditional<G == QTX_GRAM_BF16X3, __bf16, ST>::type;
+  __shared__ LT As[PLANES][GT * LD];
+  __shared__ LT Bs[PLANES][GT * LD];
+
+  int bi, bj;
+  tile_of(blockIdx.x, bi, bj);
+  const int64_t i0 = (int64_t)bi * GT, j0 = (int64_t)bj * GT;
+  const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
+  const int wr = (wv / 2) * 64, wc = (wv % 2) * 64;  // wave's sub-tile origin
+
+  using ACC = typename std::conditional<G == QTX_GRAM_FP64, v4d, v4f>::type;
+  ACC acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = ACC{0, 0, 0, 0};
+
+  for (int64_t k0 = 0; k0 < np; k0 += KC) {
+    // stage A rows [i0, i0+128) and B rows [j0, j0+128), columns [k0, k0+KC)
+    for (int e = tid; e < GT * KC; e += 256) {
+      int row = e / KC, kk = e % KC;
+      int64_t k = k0 + kk;
+#pragma unroll
+      for (int which = 0; which < 2; ++which) {
+        int64_t gr = (which == 0 ? i0 : j0) + row;
+        double v = 0.0;
+        if (gr < n && k < np) {
+          double sc = sqrt((r ? r[gr] : 1.0) * inv_ns);
+          v = ((double)O[gr * np + k] - mean[k]) * sc;
+        }
+        LT* dst = (which == 0) ? As[0] : Bs[0];
+        if constexpr (G == QTX_GRAM_BF16X3) {
+          float f = (float)v;
+          __bf16 hi = (__bf16)f;
+          __bf16 lo = (__bf16)(f - (float)hi);
+          dst[row * LD + kk] = hi;
+          ((which == 0) ? As[1] : Bs[1])[row * LD + kk] = lo;
+        } else {
+          dst[row * LD + kk] = (ST)v;
+        }
+      }
+    }
+    __syncthreads();
+    if constexpr (G == QTX_GRAM_BF16X3) {
+      // one K=32 step: 3 MFMAs per 16x16 block
+      v8bf ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int row = wr + a * 16 + lane % 16, kq = 8 * (lane / 16);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          ah[a][q] = As[0][row * LD + kq + q];
+          al[a][q] = As[1][row * LD + kq + q];
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int row = wc + b * 16 + lane % 16, kq = 8 * (lane / 16);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          bh[b][q] = Bs[0][row * LD + kq + q];
+          bl[b][q] = Bs[1][row * LD + kq + q];
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+        }
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < KC; ks += 4) {
+        ST av[4], bv[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) av[a] = As[0][(wr + a * 16 + lane % 16) * LD + ks + lane / 16];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) bv[b] = Bs[0][(wc + b * 16 + lane % 16) * LD + ks + lane / 16];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            if constexpr (G == QTX_GRAM_FP64)
+              acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+            else
+              acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv[b], acc[a][b], 0, 0, 0);
+          }
+      }
+    }
+    __syncthreads();
+  }
+  // store, column lane%16 of each 16x16 block; rows of element q:
+  //   f32 / bf16 MFMA: 4 * (lane / 16) + q ;  f64 MFMA: lane / 16 + 4 * q
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int rq = (G == QTX_GRAM_FP64) ? (lane / 16 + 4 * q) : (4 * (lane / 16) + q);
+        int64_t gi = i0 + wr + a * 16 + rq;
+        int64_t gj = j0 + wc + b * 16 + lane % 16;
+        if (gi < n && gj < n) {
+          double v = (double)acc[a][b][q];
+          T[gi * n + gj] = v;
+          if (bi != bj) T[gj * n + gi] = v;
+        }
+      }
+}
+
+// S = Obar^T Obar (np x np), FP64, small np (SR branch, ns >= np)
+template <typename TI>
+__global__ __launch_bounds__(256) void gram_tn_kernel(const TI* __restrict__ O, const double* __restrict__ mean,
+                                                      const double* __restrict__ r, int64_t ns, int64_t np,
+                                                      double inv_ns, double* __restrict__ S) {
+  __shared__ double As[32][17], Bs[32][17];
+  int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  int64_t k = (int64_t)blockIdx.y * 16 + ty, l = (int64_t)blockIdx.x * 16 + tx;
+  double acc = 0.0;
+  for (int64_t s0 = 0; s0 < ns; s0 += 32) {
+    for (int e = threadIdx.x; e < 32 * 16; e += 256) {
+      int ss = e / 16, c = e % 16;
+      int64_t s = s0 + ss;
+      int64_t ka = (int64_t)blockIdx.y * 16 + c, kb = (int64_t)blockIdx.x * 16 + c;
+      double sc = (s < ns) ? sqrt((r ? r[s] : 1.0) * inv_ns) : 0.0;
+      As[ss][c] = (s < ns && ka < np) ? ((double)O[s * np + ka] - mean[ka]) * sc : 0.0;
+      Bs[ss][c] = (s < ns && kb < np) ? ((double)O[s * np + kb] - mean[kb]) * sc : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int ss = 0; ss < 32; ++ss) acc += As[ss][ty] * Bs[ss][tx];
+    __syncthreads();
+  }
+  if (k < np && l < np) S[k * np + l] = acc;
+}
+
+// x_k = sum_s (O_sk - mean_k) sqrt(r_s / ns) y_s  (FP64 accumulation)
+template <typename TI>
+__global__ __launch_bounds__(256) void obar_t_y_kernel(const TI* __restrict__ O, const double* __restrict__ mean,
+                                                       const double* __restrict__ r, const double* __restrict__ y,
+                                                       int64_t ns, int64_t np, int64_t rows_per_chunk,
+                                                       double inv_ns, double* __restrict__ x) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  int64_t r0 = (int64_t)blockIdx.y * rows_per_chunk;
+  int64_t r1 = r0 + rows_per_chunk < ns ? r0 + rows_per_chunk : ns;
+  double acc = 0.0, m = mean[k];
+  for (int64_t s = r0; s < r1; ++s) {
+    double w = y[s] * sqrt((r ? r[s] : 1.0) * inv_ns);
+    acc += ((double)O[s * np + k] - m) * w;
+  }
+  atomicAdd(x + k, acc);
+}
+
+// single-block: scalars[0] = trace, then T_ii += rshift * trace + ashift
+__global__ __launch_bounds__(1024) void shift_diag_kernel(double* __restrict__ T, int64_t n, double rshift,
+                                                          double ashift, double* __restrict__ scalars) {
+  __shared__ double red[16];
+  int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE, nw = blockDim.x / WAVE;
+  double acc = 0.0;
+  for (int64_t i = tid; i < n; i += blockDim.x) acc += T[i * n + i];
+  acc = wave_sum(acc);
+  if (lane == 0) red[wv] = acc;
+  __syncthreads();
+  double tr = 0.0;
+  for (int w = 0; w < nw; ++w) tr += red[w];
+  double shift = rshift * tr + ashift;
+  for (int64_t i = tid; i < n; i += blockDim.x) T[i * n + i] += shift;
+  if (tid == 0) scalars[0] = tr;
+}
+
+// soft inverse of eigenvalues times rho (solver.py:94-101), single block
+__global__ __launch_bounds__(1024) void eig_inv_kernel(const double* __restrict__ vals, int64_t n, double rtol,
+                                                       double atol, double* __restrict__ rho) {
+  __shared__ double red[16];
+  int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE, nw = blockDim.x / WAVE;
+  double m = 0.0;
+  for (int64_t i = tid; i < n; i += blockDim.x) m = fmax(m, fabs(vals[i]));
+  m = wave_max(m);
+  if (lane == 0) red[wv] = m;
+  __syncthreads();
+  double vmax = 0.0;
+  for (int w = 0; w < nw; ++w) vmax = fmax(vmax, red[w]);
+  double cut = rtol * vmax + atol;
+  for (int64_t i = tid; i < n; i += blockDim.x) {
+    double v = vals[i], va = fabs(v);
+    double inv = 0.0;
+    if (va > 0.0) {
+      double q = cut / va;
+      double q2 = q * q;
+      inv = 1.0 / (v * (1.0 + q2 * q2 * q2));
+    }
+    rho[i] *= inv;
+  }
+}
+
+__global__ void copy_f64_kernel(const double* __restrict__ a, double* __restrict__ b, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void finite_check_kernel(const double* __restrict__ step, int64_t n,
+                                                           int32_t* __restrict__ ok) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool bad = (i < n) && !isfinite(step[i]);
+  if (__any(bad) && (threadIdx.x % WAVE) == 0) atomicAnd(ok, 0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void apply_update_kernel(T* __restrict__ theta, const double* __restrict__ step,
+                                                           double lr, int64_t n, const int32_t* __restrict__ ok) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || *ok == 0) return;
+  T d = (T)(lr * step[i]);
+  theta[i] = theta[i] - d;
+}
+
+// ---------------------------------------------------------------- rocBLAS handle
+static std::mutex g_blas_mu;
+static rocblas_handle g_blas[64] = {nullptr};
+
+static int get_blas(hipStream_t st, rocblas_handle* h) {
+  int dev = 0;
+  QTX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(QTX_EHIP, "device index out of range");
+  if (!g_blas[dev]) {
+    if (rocblas_create_handle(&g_blas[dev]) != rocblas_status_success)
+      return fail(QTX_ESOLVER, "rocblas_create_handle failed");
+  }
+  if (rocblas_set_stream(g_blas[dev], st) != rocblas_status_success)
+    return fail(QTX_ESOLVER, "rocblas_set_stream failed");
+  *h = g_blas[dev];
+  return QTX_OK;
+}
+
+static int rb(rocblas_status s, const char* what) {
+  if (s == rocblas_status_success) return QTX_OK;
+  return fail(QTX_ESOLVER, std::string(what) + " failed: " + rocblas_status_to_string(s));
+}
+
+}  // namespace qtx
+
+using namespace qtx;
+
+static int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+struct SolveWs {
+  double* T;       // n x n
+  double* y;       // max(ns, np)
+  double* z;       // max(ns, np)
+  double* D;       // n
+  double* E;       // n
+  double* scalars; // 8
+  int32_t* info;   // 1 (+pad)
+};
+
+static size_t ws_layout(int64_t ns, int64_t np, SolveWs* w, char* base) {
+  int64_t n = ns < np ? ns : np;
+  int64_t m = ns > np ? ns : np;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return base ? base + o : nullptr;
+  };
+  double* T = (double*)take((size_t)n * n * 8);
+  double* y = (double*)take((size_t)m * 8);
+  double* z = (double*)take((size_t)m * 8);
+  double* D = (double*)take((size_t)n * 8);
+  double* E = (double*)take((size_t)n * 8);
+  double* sc = (double*)take(64);
+  int32_t* info = (int32_t*)take(64);
+  if (w) *w = SolveWs{T, y, z, D, E, sc, info};
+  return off;
+}
+
+template <typename TI>
+static int launch_gram_nt(hipStream_t st, int gram, const TI* O, const double* mean, const double* r, int64_t n,
+                          int64_t np, double* T) {
+  int64_t nt = (n + GT - 1) / GT;
+  int64_t ntiles = nt * (nt + 1) / 2;
+  QTX_REQUIRE(ntiles < 0x7fffffff, "Gram too large");
+  dim3 g((unsigned)ntiles), bl(256);
+  double inv_ns = 1.0 / (double)n;
+  if (gram == QTX_GRAM_BF16X3)
+    hipLaunchKernelGGL((gram_nt_kernel<TI, QTX_GRAM_BF16X3>), g, bl, 0, st, O, mean, r, n, np, inv_ns, T);
+  else if (gram == QTX_GRAM_FP32)
+    hipLaunchKernelGGL((gram_nt_kernel<TI, QTX_GRAM_FP32>), g, bl, 0, st, O, mean, r, n, np, inv_ns, T);
+  else
+    hipLaunchKernelGGL((gram_nt_kernel<TI, QTX_GRAM_FP64>), g, bl, 0, st, O, mean, r, n, np, inv_ns, T);
+  QTX_LAUNCH_CHECK();
+  return QTX_OK;
+}
+
+template <typename TI>
+static int launch_obar_t_y(hipStream_t st, const TI* O, const double* mean, const double* r, const double* y,
+                           int64_t ns, int64_t np, double* x) {
+  hipLaunchKernelGGL(fill_f64_kernel, dim3(nblk(np, 256)), dim3(256), 0, st, x, np, 0.0);
+  int64_t chunks = (ns + 255) / 256;
+  if (chunks > 256) chunks = 256;
+  int64_t rpc = (ns + chunks - 1) / chunks;
+  chunks = (ns + rpc - 1) / rpc;
+  dim3 g(nblk(np, 256), (unsigned)chunks);
+  hipLaunchKernelGGL(obar_t_y_kernel<TI>, g, dim3(256), 0, st, O, mean, r, y, ns, np, rpc, 1.0 / (double)ns, x);
+  QTX_LAUNCH_CHECK();
+  return QTX_OK;
+}
+
+template <typename TI>
+static int minsr_solve_t(hipStream_t st, const TI* O, const double* mean, const double* r, const double* ebar,
+                         int64_t ns, int64_t np, int mode, int gram, double rshift, double ashift, double rtol,
+                         double atol, double* step, SolveWs& w) {
+  std::lock_guard<std::mutex> lock(g_blas_mu);
+  rocblas_handle h;
+  int rc = get_blas(st, &h);
+  if (rc) return rc;
+  const bool minnorm = ns < np;
+  const int64_t n = minnorm ? ns : np;
+  QTX_REQUIRE(n <= 0x7fffffff, "matrix too large for rocSOLVER");
+  // Gram
+  if (minnorm) {
+    rc = launch_gram_nt<TI>(st, gram, O, mean, r, ns, np, w.T);
+    if (rc) return rc;
+    hipLaunchKernelGGL(copy_f64_kernel, dim3(nblk(ns, 256)), dim3(256), 0, st, ebar, w.y, ns);
+  } else {
+    dim3 g(nblk(np, 16), nblk(np, 16));
+    hipLaunchKernelGGL(gram_tn_kernel<TI>, g, dim3(256), 0, st, O, mean, r, ns, np, 1.0 / (double)ns, w.T);
+    QTX_LAUNCH_CHECK();
+    rc = launch_obar_t_y<TI>(st, O, mean, r, ebar, ns, np, w.y);  // F = Obar^T ebar
+    if (rc) return rc;
+  }
+  const rocblas_int ni = (rocblas_int)n;
+  if (mode == QTX_SOLVE_CHOL_SHIFT) {
+    hipLaunchKernelGGL(shift_diag_kernel, dim3(1), dim3(1024), 0, st, w.T, n, rshift, ashift, w.scalars);
+    QTX_LAUNCH_CHECK();
+    rc = rb(rocsolver_dpotrf(h, rocblas_fill_lower, ni, w.T, ni, w.info), "rocsolver_dpotrf");
+    if (rc) return rc;
+    rc = rb(rocsolver_dpotrs(h, rocblas_fill_lower, ni, 1, w.T, ni, w.y, ni), "rocsolver_dpotrs");
+    if (rc) return rc;
+  } else {
+    rc = rb(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, ni, w.T, ni, w.D, w.E, w.info),
+            "rocsolver_dsyevd");
+    if (rc) return rc;
+    const double one = 1.0, zero = 0.0;
+    rocblas_pointer_mode pm;
+    rocblas_get_pointer_mode(h, &pm);
+    rocblas_set_pointer_mode(h, rocblas_pointer_mode_host);
+    // z = U^T y ; z *= eig_inv ; y = U z   (U column-major in T)
+    rc = rb(rocblas_dgemv(h, rocblas_operation_transpose, ni, ni, &one, w.T, ni, w.y, 1, &zero, w.z, 1),
+            "rocblas_dgemv");
+    if (rc) return rc;
+    hipLaunchKernelGGL(eig_inv_kernel, dim3(1), dim3(1024), 0, st, w.D, n, rtol, atol, w.z);
+    QTX_LAUNCH_CHECK();
+    rc = rb(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &one, w.T, ni, w.z, 1, &zero, w.y, 1),
+            "rocblas_dgemv");
+    rocblas_set_pointer_mode(h, pm);
+    if (rc) return rc;
+  }
+  int32_t info_h = 0;
+  QTX_HIP(hipMemcpyAsync(&info_h, w.info, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  QTX_HIP(hipStreamSynchronize(st));
+  if (info_h != 0)
+    return fail(QTX_ESOLVER, std::string(mode == QTX_SOLVE_CHOL_SHIFT ? "potrf" : "syevd") +
+                                 " info = " + std::to_string(info_h));
+  if (minnorm) {
+    rc = launch_obar_t_y<TI>(st, O, mean, r, w.y, ns, np, step);
+    if (rc) return rc;
+  } else {
+    hipLaunchKernelGGL(copy_f64_kernel, dim3(nblk(np, 256)), dim3(256), 0, st, w.y, step, np);
+    QTX_LAUNCH_CHECK();
+  }
+  return QTX_OK;
+}
+
+extern "C" {
+
+int qtx_colmean(void* stream, int o_dtype, const void* O, int64_t ns, int64_t np, double* mean) {
+  QTX_REQUIRE(o_dtype == QTX_F32 || o_dtype == QTX_F64, "o_dtype must be QTX_F32 or QTX_F64");
+  QTX_REQUIRE(ns > 0 && np > 0 && O && mean, "bad arguments");
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(fill_f64_kernel, dim3(nblk(np, 256)), dim3(256), 0, st, mean, np, 0.0);
+  int64_t chunks = (ns + 127) / 128;
+  if (chunks > 128) chunks = 128;
+  int64_t rpc = (ns + chunks - 1) / chunks;
+  chunks = (ns + rpc - 1) / rpc;
+  dim3 g(nblk(np, 256), (unsigned)chunks);
+  if (o_dtype == QTX_F32)
+    hipLaunchKernelGGL(colsum_kernel<float>, g, dim3(256), 0, st, (const float*)O, ns, np, rpc, mean);
+  else
+    hipLaunchKernelGGL(colsum_kernel<double>, g, dim3(256), 0, st, (const double*)O, ns, np, rpc, mean);
+  hipLaunchKernelGGL(scale_f64_kernel, dim3(nblk(np, 256)), dim3(256), 0, st, mean, np, 1.0 / (double)ns);
+  QTX_LAUNCH_CHECK();
+  return QTX_OK;
+}
+
+int qtx_ebar(void* stream, const double* eloc, const double* r, int64_t ns, double* ebar, double* out2) {
+  QTX_REQUIRE(ns > 0 && eloc && ebar && out2, "bad arguments");
+  hipLaunchKernelGGL(ebar_kernel, dim3(1), dim3(1024), 0, as_stream(stream), eloc, r, ns, ebar, out2);
+  QTX_LAUNCH_CHECK();
+  return QTX_OK;
+}
+
+size_t qtx_minsr_ws_bytes(int64_t ns, int64_t np, int mode, int gram) {
+  (void)mode;
+  (void)gram;
+  if (ns <= 0 || np <= 0) return 0;
+  return ws_layout(ns, np, nullptr, nullptr);
+}
+
+int qtx_minsr_solve(void* stream, int o_dtype, const void* O, const double* mean, const double* r,
+                    const double* ebar, int64_t ns, int64_t np, int mode, int gram, double rshift, double ashift,
+                    double rtol, double atol, double* step, void* ws, size_t ws_bytes) {
+  QTX_REQUIRE(o_dtype == QTX_F32 || o_dtype == QTX_F64, "o_dtype must be QTX_F32 or QTX_F64");
+  QTX_REQUIRE(ns > 0 && np > 0 && O && mean && ebar && step, "bad arguments");
+  QTX_REQUIRE(mode == QTX_SOLVE_CHOL_SHIFT || mode == QTX_SOLVE_EIGH_PINV, "unknown solver mode");
+  QTX_REQUIRE(gram == QTX_GRAM_FP64 || gram == QTX_GRAM_FP32 || gram == QTX_GRAM_BF16X3, "unknown gram mode");
+  size_t need = qtx_minsr_ws_bytes(ns, np, mode, gram);
+  if (ws == nullptr || ws_bytes < need)
+    return fail(QTX_EWORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+  SolveWs w;
+  ws_layout(ns, np, &w, (char*)ws);
+  hipStream_t st = as_stream(stream);
+  if (o_dtype == QTX_F32)
+    return minsr_solve_t<float>(st, (const float*)O, mean, r, ebar, ns, np, mode, gram, rshift, ashift, rtol, atol,
+                                step, w);
+  return minsr_solve_t<double>(st, (const double*)O, mean, r, ebar, ns, np, mode, gram, rshift, ashift, rtol, atol,
+                               step, w);
+}
+
+int qtx_apply_update(void* stream, int dtype, void* theta, const double* step, double lr, int64_t np,
+                     int32_t* applied) {
+  QTX_REQUIRE(dtype == QTX_F32 || dtype == QTX_F64, "dtype must be QTX_F32 or QTX_F64");
+  QTX_REQUIRE(np > 0 && theta && step && applied, "bad arguments");
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, st, applied, (int64_t)1, 1);
+  hipLaunchKernelGGL(finite_check_kernel<double>, dim3(nblk(np, 256)), dim3(256), 0, st, step, np, applied);
+  if (dtype == QTX_F32)
+    hipLaunchKernelGGL(apply_update_kernel<float>, dim3(nblk(np, 256)), dim3(256), 0, st, (float*)theta, step, lr,
+                       np, applied);
+  else
+    hipLaunchKernelGGL(apply_update_kernel<double>, dim3(nblk(np, 256)), dim3(256), 0, st, (double*)theta, step,
+                       lr, np, applied);
+  QTX_LAUNCH_CHECK();
+  return QTX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,194 @@
+"""SR / MinSR optimizer and the Euler driver.
+
+Mirrors ``quantax/optimizer/sr.py:17-195`` (QNGD/SR: Ebar, Obar, solve,
+get_step, energy, VarE), the solver factories of
+``quantax/optimizer/solver.py:50-201`` and ``Euler`` of
+``quantax/optimizer/driver.py:30-39``.  A solver here is a small descriptor
+(mode, shifts, tolerances, Gram precision) executed by ``qtx_minsr_solve``:
+Ns < Np selects MinSR (T = Obar Obar^T, Ns x Ns), else SR (S = Obar^T Obar),
+exactly like the ``auto_*`` factories.  Obar is formed on the fly on the
+device; nothing of size Ns x Np is materialised beyond O itself.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from .utils import DataTracer
+
+_GRAM = {"fp64": _lib.QTX_GRAM_FP64, "fp32": _lib.QTX_GRAM_FP32, "bf16x3": _lib.QTX_GRAM_BF16X3}
+
+
+@dataclass
+class Solver:
+    mode: int
+    rshift: float = 0.0
+    ashift: float = 0.0
+    rtol: float = 1e-12
+    atol: float = 0.0
+    gram: str = "fp64"
+    branch: str = "auto"  # "auto" | "minnorm" | "lstsq": the reference factory this came from
+
+    def check(self, ns: int, np_: int) -> None:
+        if self.branch == "minnorm" and not ns < np_:
+            raise ValueError("minnorm_* solvers need nsamples < nparams here; use the auto_* factory")
+        if self.branch == "lstsq" and ns < np_:
+            raise ValueError("lstsq_* solvers need nsamples >= nparams here; use the auto_* factory")
+
+
+def _rt(v):
+    return 1e-12 if v is None else float(v)  # _get_rtol for float64 (solver.py:12-21)
+
+
+def minnorm_shift_eig(rshift: Optional[float] = None, ashift: float = 1e-4, gram: str = "fp64") -> Solver:
+    return Solver(_lib.QTX_SOLVE_CHOL_SHIFT, rshift=_rt(rshift), ashift=ashift, gram=gram, branch="minnorm")
+
+
+def lstsq_shift_eig(rshift: Optional[float] = None, ashift: float = 1e-4, gram: str = "fp64") -> Solver:
+    return Solver(_lib.QTX_SOLVE_CHOL_SHIFT, rshift=_rt(rshift), ashift=ashift, gram=gram, branch="lstsq")
+
+
+def auto_shift_eig(rshift: Optional[float] = None, ashift: float = 1e-4, gram: str = "fp64") -> Solver:
+    return Solver(_lib.QTX_SOLVE_CHOL_SHIFT, rshift=_rt(rshift), ashift=ashift, gram=gram)
+
+
+def minnorm_pinv_eig(rtol: Optional[float] = None, atol: float = 0.0, tol_snr: float = 0.0,
+                     gram: str = "fp64") -> Solver:
+    if tol_snr != 0.0:
+        raise NotImplementedError("tol_snr > 0 is not supported")
+    return Solver(_lib.QTX_SOLVE_EIGH_PINV, rtol=_rt(rtol), atol=atol, gram=gram, branch="minnorm")
+
+
+def lstsq_pinv_eig(rtol: Optional[float] = None, atol: float = 0.0, tol_snr: float = 0.0,
+                   gram: str = "fp64") -> Solver:
+    if tol_snr != 0.0:
+        raise NotImplementedError("tol_snr > 0 is not supported")
+    return Solver(_lib.QTX_SOLVE_EIGH_PINV, rtol=_rt(rtol), atol=atol, gram=gram, branch="lstsq")
+
+
+def auto_pinv_eig(rtol: Optional[float] = None, atol: float = 0.0, tol_snr: float = 0.0,
+                  gram: str = "fp64") -> Solver:
+    if tol_snr != 0.0:
+        raise NotImplementedError("tol_snr > 0 is not supported")
+    return Solver(_lib.QTX_SOLVE_EIGH_PINV, rtol=_rt(rtol), atol=atol, gram=gram)
+
+
+def solve_step(O: torch.Tensor, Omean: torch.Tensor, ebar: torch.Tensor, solver: Solver,
+               r: Optional[torch.Tensor] = None, ws: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """step = solver(Obar, Ebar) with Obar = (O - Omean) sqrt(r / Ns) formed on the fly."""
+    ns, np_ = O.shape
+    solver.check(ns, np_)
+    od = _lib.QTX_F32 if O.dtype == torch.float32 else _lib.QTX_F64
+    need = int(_lib.lib().qtx_minsr_ws_bytes(ns, np_, solver.mode, _GRAM[solver.gram]))
+    if ws is None or ws.numel() < need:
+        ws = torch.empty(need, dtype=torch.uint8, device=O.device)
+    step = torch.empty(np_, dtype=torch.float64, device=O.device)
+    _lib.call("qtx_minsr_solve", _lib.stream(), od, O.data_ptr(), Omean.data_ptr(), _lib.ptr(r), ebar.data_ptr(),
+              ns, np_, solver.mode, _GRAM[solver.gram], solver.rshift, solver.ashift, solver.rtol, solver.atol,
+              step.data_ptr(), ws.data_ptr(), ws.numel())
+    return step
+
+
+class QNGD:
+    def __init__(self, state, imag_time: bool = True, solver: Optional[Solver] = None):
+        if not imag_time:
+            raise NotImplementedError("real-time evolution is not supported")
+        self._state = state
+        self._solver = auto_pinv_eig() if solver is None else solver
+        self._Omean = None
+        self._ws = None
+
+    @property
+    def state(self):
+        return self._state
+
+    @property
+    def solver(self) -> Solver:
+        return self._solver
+
+    def get_Ebar(self, samples):
+        raise NotImplementedError
+
+    def get_Obar(self, samples):
+        """Returns (O, Omean): Obar = (O - mean O) sqrt(r / Ns) is applied
+        inside the solve (``sr.py:74-88``)."""
+        O = self._state.jacobian(samples.spins)
+        mean = torch.empty(O.shape[1], dtype=torch.float64, device=O.device)
+        _lib.call("qtx_colmean", _lib.stream(), _lib.QTX_F32 if O.dtype == torch.float32 else _lib.QTX_F64,
+                  O.data_ptr(), O.shape[0], O.shape[1], mean.data_ptr())
+        self._Omean = mean
+        return O, mean
+
+    def get_step(self, samples) -> torch.Tensor:
+        ebar = self.get_Ebar(samples)
+        O, mean = self.get_Obar(samples)
+        r = None if samples.reweight_factor is None or self._trivial_r(samples) else samples.reweight_factor
+        ns, np_ = O.shape
+        need = int(_lib.lib().qtx_minsr_ws_bytes(ns, np_, self._solver.mode, _GRAM[self._solver.gram]))
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = torch.empty(need, dtype=torch.uint8, device=O.device)
+        return solve_step(O, mean, ebar, self._solver, r, self._ws)
+
+    @staticmethod
+    def _trivial_r(samples) -> bool:
+        return bool(torch.all(samples.reweight_factor == 1.0).item())
+
+
+class SR(QNGD):
+    """Stochastic reconfiguration, MinSR when Ns < Np (``sr.py:131-195``)."""
+
+    def __init__(self, state, hamiltonian, imag_time: bool = True, solver: Optional[Solver] = None):
+        super().__init__(state, imag_time, solver)
+        self._hamiltonian = hamiltonian
+        self._energy = None
+        self._VarE = None
+        self._eloc = None
+
+    @property
+    def hamiltonian(self):
+        return self._hamiltonian
+
+    @property
+    def energy(self) -> Optional[float]:
+        return None if self._energy is None else float(self._energy)
+
+    @property
+    def VarE(self) -> Optional[float]:
+        return None if self._VarE is None else float(self._VarE)
+
+    def get_Ebar(self, samples) -> torch.Tensor:
+        eloc = self._hamiltonian.Oloc(self._state, samples)
+        self._eloc = eloc
+        ns = eloc.shape[0]
+        r = samples.reweight_factor
+        r = None if r is None or self._trivial_r(samples) else r.contiguous()
+        ebar = torch.empty(ns, dtype=torch.float64, device=eloc.device)
+        out2 = torch.empty(2, dtype=torch.float64, device=eloc.device)
+        _lib.call("qtx_ebar", _lib.stream(), eloc.data_ptr(), _lib.ptr(r), ns, ebar.data_ptr(), out2.data_ptr())
+        self._energy, self._VarE = out2[0], out2[1]
+        return ebar
+
+
+class Euler:
+    """One VMC step per call: sweep -> get_step -> update (``driver.py:30-39``)."""
+
+    def __init__(self, state, sampler, optimizer, step_length: float):
+        self._state = state
+        self._sampler = sampler
+        self._optimizer = optimizer
+        self._step_length = step_length
+        self._time = 0.0
+        self.energy = DataTracer()
+        self.VarE = DataTracer()
+
+    def step(self) -> None:
+        samples = self._sampler.sweep()
+        step = self._optimizer.get_step(samples)
+        self._state.update(self._step_length * step)
+        self._time += self._step_length
+        self.energy.append(self._optimizer.energy, self._time)
+        self.VarE.append(self._optimizer.VarE, self._time)

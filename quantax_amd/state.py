@@ -1,0 +1,92 @@
+"""Variational state: ``quantax/state/variational.py`` for real parameters.
+
+``__call__`` returns log psi as a ``LogArray`` (the reference returns a
+PsiArray cast to float64, ``variational.py:261-347``); ``jacobian`` returns
+O = d log|psi| / d theta per sample in ravel_pytree order
+(``variational.py:424-511``); ``update`` applies theta <- theta - step and
+skips non-finite steps with a warning (``variational.py:558-579``).
+``max_parallel`` is accepted for signature compatibility: the kernels tile
+samples internally, so results never depend on it.
+"""
+from __future__ import annotations
+
+from typing import Optional
+from warnings import warn
+
+import numpy as np
+import torch
+
+from . import _lib
+from .utils import LogArray
+
+
+class Variational:
+    def __init__(self, model, param_file: Optional[str] = None, symm=None, max_parallel=None, use_ref: bool = True):
+        if symm is not None:
+            raise NotImplementedError("only the identity symmetry is supported")
+        self._model = model
+        self._max_parallel = max_parallel
+        self._use_ref = use_ref
+        if param_file is not None:
+            self.load(param_file)
+
+    @property
+    def model(self):
+        return self._model
+
+    @property
+    def nparams(self) -> int:
+        return self._model.nparams
+
+    @property
+    def Nsites(self) -> int:
+        return self._model.N
+
+    @property
+    def Nmodes(self) -> int:
+        return self._model.N
+
+    @property
+    def dtype(self):
+        return self._model.theta.dtype
+
+    @property
+    def use_ref(self) -> bool:
+        return self._use_ref
+
+    def __call__(self, s) -> LogArray:
+        s = torch.as_tensor(s, device=_lib.device()).to(torch.int8).reshape(-1, self.Nmodes)
+        la = self._model.logpsi(s)
+        return LogArray(torch.ones_like(la), la)
+
+    def jacobian(self, fock_states) -> torch.Tensor:
+        s = torch.as_tensor(fock_states, device=_lib.device()).to(torch.int8).reshape(-1, self.Nmodes)
+        return self._model.jacobian(s)
+
+    def get_params_flatten(self) -> torch.Tensor:
+        return self._model.theta.clone()
+
+    def set_params(self, theta) -> None:
+        self._model.set_params(theta)
+
+    def update(self, step: torch.Tensor) -> bool:
+        """theta <- theta - step; returns False (and warns) when skipped."""
+        step = torch.as_tensor(step, device=_lib.device()).to(torch.float64).contiguous()
+        if step.numel() != self.nparams:
+            raise ValueError("step size does not match the number of parameters")
+        flag = torch.empty(1, dtype=torch.int32, device=step.device)
+        m = self._model
+        _lib.call("qtx_apply_update", _lib.stream(), _lib.QTX_F32 if m.theta.dtype == torch.float32 else _lib.QTX_F64,
+                  m.theta.data_ptr(), step.data_ptr(), 1.0, self.nparams, flag.data_ptr())
+        ok = bool(flag.item())
+        if not ok:
+            warn("Got invalid update step. The update is interrupted.")
+            return False
+        m.repack()
+        return True
+
+    def save(self, file) -> None:
+        np.save(file, self._model.theta.cpu().numpy(), allow_pickle=False)
+
+    def load(self, file) -> None:
+        self._model.set_params(np.load(file, allow_pickle=False))

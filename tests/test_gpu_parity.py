@@ -1,0 +1,318 @@
+"""HIP path vs the CPU oracle on the same seeded inputs (needs an MI355X).
+
+Tolerances (north star): float32 network paths 1e-5 relative, float64 paths
+1e-10 relative; integer/byte work (streams, initial chains, connected
+configurations, chains of the float64 RBM) bit-exact.  Relative errors of
+log-amplitudes and local energies use an absolute floor of 1 so that values
+crossing zero are not over-weighted.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import models as om
+from oracle import operator as oop
+from oracle import philox
+from oracle import sampler as osm
+from oracle import sr as osr
+from tests.helpers import max_rel, rand_spins
+
+pytestmark = pytest.mark.gpu
+
+import quantax_amd as qtx  # noqa: E402
+from quantax_amd import _lib  # noqa: E402
+
+DEV = "cuda"
+
+
+def to_dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+# ------------------------------------------------------------------ streams
+def test_philox_fill_bit_exact():
+    words = torch.empty((7, 33, 4), dtype=torch.int32, device=DEV)
+    _lib.call("qtx_philox_fill", _lib.stream(), 123456789123, _lib.TAG_SWEEP, 5, 7, 0, 33, words.data_ptr())
+    got = words.cpu().numpy().view(np.uint32)
+    assert np.array_equal(got, philox.sweep_words(123456789123, 7, 33, step0=5))
+
+
+@pytest.mark.parametrize("N,nup", [(16, None), (100, 50), (37, 11), (300, None)])
+def test_init_spins_bit_exact(N, nup):
+    s = torch.empty((65, N), dtype=torch.int8, device=DEV)
+    _lib.call("qtx_init_spins", _lib.stream(), 987654321, 65, N, -1 if nup is None else nup, s.data_ptr())
+    got = s.cpu().numpy()
+    ref = philox.init_spins(987654321, 65, N, nup)
+    assert np.array_equal(got, ref)
+    if nup is not None:
+        assert np.all(got.sum(1) == 2 * nup - N)
+
+
+# ------------------------------------------------------------------ operator
+@pytest.mark.parametrize("case", ["j1j2_4x4", "heis_6x6", "ising_chain16", "heis_chain10"])
+def test_connected_configs_bit_exact(case, rng):
+    if case == "j1j2_4x4":
+        qtx.sites.Square(4)
+        H = qtx.operator.Heisenberg(J=[1, 0.5], n_neighbor=[1, 2], msr=True)
+        Ho = oop.heisenberg([4, 4], [1, 0.5], [1, 2], True)
+    elif case == "heis_6x6":
+        qtx.sites.Square(6)
+        H = qtx.operator.Heisenberg()
+        Ho = oop.heisenberg([6, 6])
+    elif case == "ising_chain16":
+        qtx.sites.Chain(16)
+        H = qtx.operator.Ising(h=0.7, J=1.3)
+        Ho = oop.ising([16], h=0.7, J=1.3)
+    else:
+        qtx.sites.Chain(10)
+        H = qtx.operator.Heisenberg(msr=True)
+        Ho = oop.heisenberg([10], msr=True)
+    N = qtx.get_sites().Nsites
+    s = rand_spins(rng, 50, N)
+    diag = H.apply_diag(to_dev(s)).cpu().numpy()
+    # FP64 sums of the diagonal terms: exact for dyadic couplings, otherwise
+    # equal up to the summation order
+    assert np.allclose(diag, oop.apply_diag(s, Ho), rtol=0, atol=1e-12)
+    seg, cand, sc, hc = (x.cpu().numpy() for x in H.connected(to_dev(s)))
+    offd = oop.apply_off_diag(s, Ho)
+    assert len(offd) == 1
+    (sc_ref, H_ref), = offd.values()
+    seg_ref, sconn_ref, h_ref = oop.get_conn(sc_ref, H_ref)
+    assert np.array_equal(seg, seg_ref)
+    assert np.array_equal(sc, sconn_ref)
+    assert np.array_equal(hc, h_ref)
+
+
+# ------------------------------------------------------------------ RBM
+def _rbm(L, M, dtype, seed=3, nup=None, chain=False):
+    lat = qtx.sites.Chain(L) if chain else qtx.sites.Square(L, Nparticles=None if nup is None else (nup, L * L - nup))
+    model = qtx.model.RBM_Dense(M, dtype=dtype, init_std=0.15, seed=seed)
+    theta = model.theta.cpu().numpy().astype(np.float64)
+    W, b = om.rbm_unpack(theta, M, lat.Nsites)
+    return lat, model, W, b
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float32, 1e-5), (np.float64, 1e-10)])
+@pytest.mark.parametrize("M", [12, 64, 200, 700])
+def test_rbm_logpsi(dtype, tol, M, rng):
+    lat, model, W, b = _rbm(6, M, dtype)
+    s = rand_spins(rng, 40, lat.Nsites)
+    la = model.logpsi(to_dev(s)).cpu().numpy()
+    assert max_rel(la, om.rbm_logpsi(W, b, s)[1]) < tol
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float32, 2e-6), (np.float64, 1e-12)])
+def test_rbm_jacobian(dtype, tol, rng):
+    lat, model, W, b = _rbm(4, 24, dtype)
+    s = rand_spins(rng, 30, lat.Nsites)
+    O = model.jacobian(to_dev(s)).cpu().numpy()
+    assert O.shape == (30, 24 * 16 + 24)
+    assert np.max(np.abs(O - om.rbm_jacobian(W, b, s))) < tol
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float32, 1e-5), (np.float64, 1e-10)])
+def test_rbm_eloc_j1j2(dtype, tol, rng):
+    lat, model, W, b = _rbm(4, 32, dtype)
+    H = qtx.operator.Heisenberg(J=[1, 0.5], n_neighbor=[1, 2], msr=True)
+    Ho = oop.heisenberg([4, 4], [1, 0.5], [1, 2], True)
+    s = rand_spins(rng, 64, 16, nup=8)
+    e, nconn = model.eloc(H, to_dev(s), None)
+    ref = oop.oloc(s, lambda x: om.rbm_logpsi(W, b, x), Ho)
+    assert max_rel(e.cpu().numpy(), ref) < tol
+    seg, _, _ = oop.get_conn(*list(oop.apply_off_diag(s, Ho).values())[0])
+    assert np.array_equal(nconn.cpu().numpy(), np.bincount(seg, minlength=64))
+
+
+def test_rbm_eloc_ising_f64(rng):
+    lat, model, W, b = _rbm(16, 16, np.float64, chain=True)
+    H = qtx.operator.Ising(h=1.0)
+    Ho = oop.ising([16], h=1.0)
+    s = rand_spins(rng, 128, 16)
+    e, _ = model.eloc(H, to_dev(s), None)
+    ref = oop.oloc(s, lambda x: om.rbm_logpsi(W, b, x), Ho)
+    assert max_rel(e.cpu().numpy(), ref) < 1e-10
+
+
+@pytest.mark.parametrize("kind", ["flip", "exchange"])
+def test_rbm_metropolis_chains_f64_bit_identical(kind, rng):
+    nup = 18 if kind == "exchange" else None
+    lat, model, W, b = _rbm(6, 40, np.float64, nup=nup)
+    N = 36
+    ns, nsteps = 96, 3 * N
+    s0 = rand_spins(rng, ns, N, nup)
+    words = philox.sweep_words(77, nsteps, ns)
+    if kind == "flip":
+        kd, nbr, width, hop = _lib.QTX_LOCALFLIP, None, 0, 1
+        nbr_np = None
+    else:
+        kd = _lib.QTX_EXCHANGE
+        nbr_np = lat.neighbor_table(1)
+        nbr, width, hop = to_dev(nbr_np), nbr_np.shape[1], 1
+    s = to_dev(s0)
+    la = model.logpsi(s)
+    nacc = torch.zeros(ns, dtype=torch.int32, device=DEV)
+    nnear = torch.zeros(ns, dtype=torch.int32, device=DEV)
+    model.metropolis(kd, nbr, width, hop, to_dev(words.view(np.int32)), nsteps, s, la, nacc, nnear, 2.0, 1e-9)
+    s_ref, la_ref, nacc_ref, near_ref = osm.metropolis(
+        lambda x: om.rbm_logpsi(W, b, x), s0, words, osm.LOCALFLIP if kind == "flip" else osm.EXCHANGE,
+        nbr_np, 1, 2.0, 1e-9)
+    assert near_ref.sum() == 0
+    assert np.array_equal(s.cpu().numpy(), s_ref)
+    assert np.array_equal(nacc.cpu().numpy(), nacc_ref)
+    assert max_rel(la.cpu().numpy(), la_ref) < 1e-10
+    assert 0 < nacc_ref.sum() < ns * nsteps
+
+
+def test_rbm_metropolis_f32_identical_outside_ties(rng):
+    lat, model, W, b = _rbm(6, 40, np.float32)
+    N, ns, nsteps = 36, 128, 72
+    s0 = rand_spins(rng, ns, N)
+    words = philox.sweep_words(5, nsteps, ns)
+    s = to_dev(s0)
+    la = model.logpsi(s)
+    nacc = torch.zeros(ns, dtype=torch.int32, device=DEV)
+    nnear = torch.zeros(ns, dtype=torch.int32, device=DEV)
+    tie = 1e-4
+    model.metropolis(_lib.QTX_LOCALFLIP, None, 0, 1, to_dev(words.view(np.int32)), nsteps, s, la, nacc, nnear, 2.0,
+                     tie)
+    s_ref, la_ref, _, near_ref = osm.metropolis(lambda x: om.rbm_logpsi(W, b, x), s0, words, osm.LOCALFLIP,
+                                                tie_tol=tie)
+    clean = (near_ref == 0) & (nnear.cpu().numpy() == 0)
+    assert clean.sum() >= ns - 4
+    assert np.array_equal(s.cpu().numpy()[clean], s_ref[clean])
+    assert max_rel(la.cpu().numpy()[clean], la_ref[clean]) < 1e-5
+
+
+# ------------------------------------------------------------------ ResConv
+def _resconv(L, nb, C, precision="fp32", seed=11, gelu="tanh", nup=None):
+    lat = qtx.sites.Square(L, Nparticles=None if nup is None else (nup, L * L - nup))
+    model = qtx.model.ResConv(nb, C, precision=precision, gelu=gelu, seed=seed)
+    # random non-zero biases so every parameter block is exercised
+    th = model.theta.cpu().numpy().astype(np.float64)
+    r = np.random.default_rng(seed + 1)
+    spec = om.ResConvSpec(nb, C, (L, L))
+    for (cin, cout, hb), (w, bo) in zip(spec.layers, spec.offsets):
+        if hb:
+            th[bo: bo + cout] = 0.1 * r.standard_normal(cout)
+    model.set_params(th.astype(np.float32))
+    theta = model.theta.cpu().numpy().astype(np.float64)
+    return lat, model, spec, theta
+
+
+@pytest.mark.parametrize("L,nb,C", [(4, 2, 16), (6, 3, 16), (16, 8, 16), (10, 2, 16), (20, 2, 16), (6, 2, 32),
+                                    (20, 3, 32)])
+def test_resconv_logpsi_fp32(L, nb, C, rng):
+    lat, model, spec, theta = _resconv(L, nb, C)
+    s = rand_spins(rng, 9, L * L)
+    la = model.logpsi(to_dev(s)).cpu().numpy()
+    assert max_rel(la, om.resconv_forward(spec, theta, s)[1]) < 1e-5
+
+
+def test_resconv_logpsi_gelu_erf(rng):
+    lat, model, spec, theta = _resconv(6, 2, 16, gelu="erf")
+    s = rand_spins(rng, 9, 36)
+    la = model.logpsi(to_dev(s)).cpu().numpy()
+    assert max_rel(la, om.resconv_forward(spec, theta, s, approximate=False)[1]) < 1e-5
+
+
+@pytest.mark.parametrize("L,nb,C", [(4, 2, 16), (16, 8, 16), (6, 2, 32)])
+def test_resconv_logpsi_bf16x3(L, nb, C, rng):
+    lat, model, spec, theta = _resconv(L, nb, C, precision="bf16x3")
+    s = rand_spins(rng, 9, L * L)
+    la = model.logpsi(to_dev(s)).cpu().numpy()
+    assert max_rel(la, om.resconv_forward(spec, theta, s)[1]) < 1e-4
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-5), ("bf16x3", 1e-4)])
+def test_resconv_eloc_j1j2(precision, tol, rng):
+    lat, model, spec, theta = _resconv(4, 2, 16, precision=precision, nup=8)
+    H = qtx.operator.Heisenberg(J=[1, 0.5], n_neighbor=[1, 2], msr=True)
+    Ho = oop.heisenberg([4, 4], [1, 0.5], [1, 2], True)
+    s = rand_spins(rng, 24, 16, nup=8)
+    la = model.logpsi(to_dev(s))
+    e, _ = model.eloc(H, to_dev(s), la)
+    ref = oop.oloc(s, lambda x: om.resconv_forward(spec, theta, x), Ho)
+    assert max_rel(e.cpu().numpy(), ref) < tol
+
+
+def test_resconv_jacobian(rng):
+    lat, model, spec, theta = _resconv(6, 2, 16)
+    s = rand_spins(rng, 5, 36)
+    O = model.jacobian(to_dev(s)).cpu().numpy()
+    ref = om.resconv_jacobian(spec, theta, s)
+    assert np.max(np.abs(O - ref)) < 1e-5 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_resconv_metropolis_identical_outside_ties(rng):
+    lat, model, spec, theta = _resconv(4, 2, 16, nup=8)
+    N, ns, nsteps = 16, 16, 24
+    s0 = rand_spins(rng, ns, N, nup=8)
+    words = philox.sweep_words(9, nsteps, ns)
+    nbr_np = lat.neighbor_table(1)
+    s = to_dev(s0)
+    la = model.logpsi(s)
+    nacc = torch.zeros(ns, dtype=torch.int32, device=DEV)
+    nnear = torch.zeros(ns, dtype=torch.int32, device=DEV)
+    tie = 1e-4
+    model.metropolis(_lib.QTX_EXCHANGE, to_dev(nbr_np), nbr_np.shape[1], 1, to_dev(words.view(np.int32)), nsteps, s,
+                     la, nacc, nnear, 2.0, tie)
+    s_ref, la_ref, nacc_ref, near_ref = osm.metropolis(lambda x: om.resconv_forward(spec, theta, x), s0, words,
+                                                       osm.EXCHANGE, nbr_np, 1, 2.0, tie)
+    clean = (near_ref == 0) & (nnear.cpu().numpy() == 0)
+    assert clean.sum() >= ns - 2
+    assert np.array_equal(s.cpu().numpy()[clean], s_ref[clean])
+    assert max_rel(la.cpu().numpy()[clean], la_ref[clean]) < 1e-5
+
+
+# ------------------------------------------------------------------ SR / MinSR
+def _solver_case(ns, np_, dtype, rng):
+    O = rng.standard_normal((ns, np_)) * 0.5 + 0.3
+    O = O.astype(dtype)
+    E = rng.standard_normal(ns) - 3.0
+    return O, E
+
+
+@pytest.mark.parametrize("gram,tol", [("fp64", 1e-10), ("fp32", 2e-4), ("bf16x3", 2e-4)])
+@pytest.mark.parametrize("mode", ["chol", "eigh"])
+@pytest.mark.parametrize("shape", [(96, 300), (200, 1000), (300, 40)])
+def test_minsr_solve(gram, tol, mode, shape, rng):
+    ns, np_ = shape
+    O, E = _solver_case(ns, np_, np.float32, rng)
+    Od = to_dev(O)
+    mean = torch.empty(np_, dtype=torch.float64, device=DEV)
+    _lib.call("qtx_colmean", _lib.stream(), _lib.QTX_F32, Od.data_ptr(), ns, np_, mean.data_ptr())
+    Ebar, Em, Ev = osr.get_Ebar(E)
+    eb = torch.empty(ns, dtype=torch.float64, device=DEV)
+    out2 = torch.empty(2, dtype=torch.float64, device=DEV)
+    _lib.call("qtx_ebar", _lib.stream(), to_dev(E).data_ptr(), None, ns, eb.data_ptr(), out2.data_ptr())
+    assert max_rel(eb.cpu().numpy(), Ebar) < 1e-12
+    assert abs(out2[0].item() - Em) < 1e-12 * abs(Em) and abs(out2[1].item() - Ev) < 1e-12 * abs(Ev)
+    Obar, Omean = osr.get_Obar(O.astype(np.float64))
+    assert np.max(np.abs(mean.cpu().numpy() - Omean)) < 1e-12
+    if mode == "chol":
+        solver = qtx.optimizer.auto_shift_eig(rshift=0.0, ashift=1e-3, gram=gram)
+        ref = osr.auto_shift_eig(Obar, Ebar, rshift=0.0, ashift=1e-3)
+    else:
+        solver = qtx.optimizer.auto_pinv_eig(rtol=1e-8, gram=gram)
+        ref = osr.auto_pinv_eig(Obar, Ebar, rtol=1e-8)
+    if ns >= np_:
+        solver.gram = "fp64"  # the SR branch always forms S in FP64
+    step = qtx.optimizer.solve_step(Od, mean, eb, solver).cpu().numpy()
+    err = np.linalg.norm(step - ref) / np.linalg.norm(ref)
+    assert err < (1e-10 if ns >= np_ else tol), err
+
+
+def test_apply_update_skips_nonfinite():
+    qtx.sites.Chain(8)
+    model = qtx.model.RBM_Dense(4, dtype=np.float32, init_std=0.1, seed=1)
+    state = qtx.state.Variational(model)
+    th0 = state.get_params_flatten().cpu().numpy()
+    step = np.linspace(-1, 1, state.nparams)
+    assert state.update(torch.from_numpy(step))
+    th1 = state.get_params_flatten().cpu().numpy()
+    assert np.array_equal(th1, osr.update(th0, step))
+    bad = step.copy()
+    bad[3] = np.nan
+    with pytest.warns(UserWarning):
+        assert not state.update(torch.from_numpy(bad))
+    assert np.array_equal(state.get_params_flatten().cpu().numpy(), th1)

@@ -1,0 +1,62 @@
+"""End-to-end VMC on the device: convergence to exact-diagonalisation values
+and a full SR step against the oracle on the same samples (needs an MI355X)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import models as om
+from oracle import operator as oop
+from oracle import sr as osr
+from tests.helpers import max_rel
+
+pytestmark = pytest.mark.gpu
+
+import quantax_amd as qtx  # noqa: E402
+
+E0_TFIM16 = -2.0 / np.sin(np.pi / 32)  # -20.404594474756657, closed form
+
+
+def test_tfim16_rbm_sr_converges():
+    """TFIM L=16, h=1: RBM(16) in float64, LocalFlip, SR with diag shift 1e-3,
+    lr 1e-2 -> the energy approaches the exact ground state."""
+    qtx.set_random_seed(0)
+    qtx.sites.Chain(16)
+    H = qtx.operator.Ising(h=1.0)
+    model = qtx.model.RBM_Dense(16, dtype=np.float64, init_std=0.01, seed=0)
+    state = qtx.state.Variational(model)
+    sampler = qtx.sampler.LocalFlip(state, 1024, seed=1)
+    opt = qtx.optimizer.SR(state, H, solver=qtx.optimizer.lstsq_shift_eig(rshift=0.0, ashift=1e-3))
+    drv = qtx.optimizer.Euler(state, sampler, opt, 1e-2)
+    for _ in range(200):
+        drv.step()
+    e = drv.energy.mean(20)
+    assert abs(e - E0_TFIM16) / abs(E0_TFIM16) < 5e-3, e
+
+
+def test_sr_step_matches_oracle_on_same_samples():
+    """One MinSR step (4x4 J1-J2, RBM float64) on device samples, recomputed
+    by the oracle from the same spins: E_loc, energy and the update vector."""
+    qtx.set_random_seed(3)
+    lat = qtx.sites.Square(4, Nparticles=(8, 8))
+    H = qtx.operator.Heisenberg(J=[1, 0.5], n_neighbor=[1, 2], msr=True)
+    model = qtx.model.RBM_Dense(32, dtype=np.float64, init_std=0.1, seed=2)
+    state = qtx.state.Variational(model)
+    sampler = qtx.sampler.SpinExchange(state, 256, seed=4)
+    for solver, ref_solver in [
+        (qtx.optimizer.auto_shift_eig(rshift=0.0, ashift=1e-4),
+         lambda A, b: osr.auto_shift_eig(A, b, rshift=0.0, ashift=1e-4)),
+        (qtx.optimizer.auto_pinv_eig(), lambda A, b: osr.auto_pinv_eig(A, b)),
+    ]:
+        opt = qtx.optimizer.SR(state, H, solver=solver)
+        samples = sampler.sweep()
+        step = opt.get_step(samples).cpu().numpy()
+        s = samples.spins.cpu().numpy()
+        theta = state.get_params_flatten().cpu().numpy()
+        W, b = om.rbm_unpack(theta, 32, 16)
+        Ho = oop.heisenberg([4, 4], [1, 0.5], [1, 2], True)
+        eloc = oop.oloc(s, lambda x: om.rbm_logpsi(W, b, x), Ho)
+        O = om.rbm_jacobian(W, b, s)
+        ref, E, VarE = osr.sr_step(O, eloc, ref_solver)
+        assert abs(opt.energy - E) < 1e-10 * abs(E)
+        assert abs(opt.VarE - VarE) < 1e-8 * max(1.0, abs(VarE))
+        assert np.linalg.norm(step - ref) / np.linalg.norm(ref) < 1e-8
