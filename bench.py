@@ -1,0 +1,244 @@
+"""VMC-step benchmark on one MI355X.
+
+Workload (default): J1-J2 Heisenberg (J2 = 0.5, Marshall sign) on a 16x16
+periodic square lattice, Sz = 0; ResConv(8 blocks, 16 channels, 3x3) with
+He-normal weights; 16384 chains (one sample per chain, the reference's
+semantics) advanced by SpinExchange for 2N = 512 proposals per sweep; MinSR
+with a 16384 x 16384 T solved by FP64 eigh (soft pseudo-inverse).  One step =
+sweep -> E_loc -> O -> Gram + eigh -> update, i.e. ``Euler.step``.
+
+Modes: "mixed" = split-bf16 MFMA convolutions and split-bf16 MFMA Gram (FP32
+accumulation), FP64 centring/eigh/update; "plain" = FP32 MFMA convolutions and
+FP64 MFMA Gram.  The JSON line reports the mixed mode as ``value`` and the
+plain mode beside it.
+
+    python bench.py [--steps K] [--warmup W] [--mode mixed|plain|both] [--config c3|c2|c1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+MFMA_PEAK = {"bf16": 2500.0, "fp32": 157.3, "fp64": 78.6}  # dense TFLOP/s
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=2)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--mode", default="both", choices=["mixed", "plain", "both"])
+    p.add_argument("--config", default="c3", choices=["c3", "c2", "c1"])
+    p.add_argument("--sweep-steps", type=int, default=None, help="proposals per chain per sweep (default 2N)")
+    p.add_argument("--thermal-sweeps", type=int, default=1)
+    p.add_argument("--solver", default="eigh", choices=["eigh", "chol"])
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    return p.parse_args()
+
+
+CONFIGS = {
+    # name: (L, lattice kind, model kind, model args, ns, J, n_neighbor)
+    "c3": dict(L=16, model="resconv", nblocks=8, channels=16, ns=16384, J=[1.0, 0.5], nn=[1, 2], msr=True,
+               workload="J1-J2 16x16 (J2=0.5, MSR, Sz=0), ResConv(8 blocks, 16 ch), 16384 chains, MinSR eigh"),
+    "c2": dict(L=10, model="rbm", features=400, ns=4096, J=[1.0], nn=[1], msr=True,
+               workload="Heisenberg 10x10 (MSR, Sz=0), RBM_Dense(alpha=4), 4096 chains, MinSR"),
+    "c1": dict(L=16, model="rbm1d", features=16, ns=1024,
+               workload="TFIM chain L=16 (h=1), RBM_Dense(16) float64, 1024 chains, SR"),
+}
+
+
+def conv_flops_per_forward(nblocks, C, N):
+    """Algorithmic FLOPs of one ResConv forward (multiply-add = 2)."""
+    f = 2 * C * 1 * 9 * N  # first conv (Cin = 1)
+    f += (2 * nblocks - 1) * 2 * C * C * 9 * N
+    return f
+
+
+class Stage:
+    """HIP-event timer on the current torch stream."""
+
+    def __init__(self, torch):
+        self.torch = torch
+        self.t = {}
+
+    def run(self, name, fn):
+        t = self.torch
+        a, b = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        a.record()
+        out = fn()
+        b.record()
+        self.t.setdefault(name, []).append((a, b))
+        return out
+
+    def ms(self):
+        self.torch.cuda.synchronize()
+        return {k: float(np.mean([a.elapsed_time(b) for a, b in v])) for k, v in self.t.items()}
+
+
+def build_problem(cfg, mode, args, qtx):
+    L = cfg["L"]
+    qtx.set_random_seed(0)
+    if cfg["model"] == "rbm1d":
+        lat = qtx.sites.Chain(L)
+        H = qtx.operator.Ising(h=1.0)
+        model = qtx.model.RBM_Dense(cfg["features"], dtype=np.float64, init_std=0.01, seed=0)
+        state = qtx.state.Variational(model)
+        sampler = qtx.sampler.LocalFlip(state, cfg["ns"], thermal_steps=args.thermal_sweeps * 2 * L, seed=1,
+                                        sweep_steps=args.sweep_steps)
+        solver = qtx.optimizer.lstsq_shift_eig(rshift=0.0, ashift=1e-3)
+        return lat, H, state, sampler, solver
+    N = L * L
+    lat = qtx.sites.Square(L, Nparticles=(N // 2, N // 2))
+    H = qtx.operator.Heisenberg(J=cfg["J"], n_neighbor=cfg["nn"], msr=cfg["msr"])
+    gram = "bf16x3" if mode == "mixed" else "fp64"
+    if cfg["model"] == "resconv":
+        model = qtx.model.ResConv(cfg["nblocks"], cfg["channels"], precision="bf16x3" if mode == "mixed" else "fp32",
+                                  seed=0)
+    else:
+        model = qtx.model.RBM_Dense(cfg["features"], dtype=np.float32, init_std=0.01, seed=0)
+        gram = "bf16x3" if mode == "mixed" else "fp64"
+    state = qtx.state.Variational(model)
+    sampler = qtx.sampler.SpinExchange(state, cfg["ns"], thermal_steps=args.thermal_sweeps * 2 * N,
+                                       sweep_steps=args.sweep_steps, seed=1)
+    if args.solver == "eigh":
+        solver = qtx.optimizer.auto_pinv_eig(gram=gram)
+    else:
+        solver = qtx.optimizer.auto_shift_eig(rshift=0.0, ashift=1e-4, gram=gram)
+    return lat, H, state, sampler, solver
+
+
+def run_mode(cfg, mode, args, torch, qtx):
+    lat, H, state, sampler, solver = build_problem(cfg, mode, args, qtx)
+    opt = qtx.optimizer.SR(state, H, solver=solver)
+    stages = Stage(torch)
+    lr = 1e-2
+
+    def vmc_step():
+        samples = stages.run("sweep", sampler.sweep)
+        ebar = stages.run("eloc", lambda: opt.get_Ebar(samples))
+        O, mean = stages.run("jacobian", lambda: opt.get_Obar(samples))
+        step = stages.run("solve", lambda: qtx.optimizer.solve_step(O, mean, ebar, solver, None, opt._ws))
+        stages.run("update", lambda: state.update(lr * step))
+        return opt
+
+    for _ in range(args.warmup):
+        vmc_step()
+        print(f"[bench] {mode}: warmup step done, E = {opt.energy:.6f}", file=sys.stderr, flush=True)
+    stages.t.clear()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    energies = []
+    for i in range(args.steps):
+        vmc_step()
+        energies.append(opt.energy)
+        print(f"[bench] {mode}: timed step {i + 1}/{args.steps}, E = {opt.energy:.6f}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return dict(ms_per_step=1e3 * dt / args.steps, steps_per_s=args.steps / dt, stages=stages.ms(),
+                energies=energies, acceptance=sampler.acceptance, nconn=float(H._connectivity.float().mean().item()),
+                nparams=state.nparams, sampler=sampler, state=state, H=H)
+
+
+def cpu_baseline(cfg, seconds):
+    """Time the float64 oracle on a bounded sample of the workload on the host
+    cores and convert it to VMC steps/s (forwards dominate: sweep + E_loc)."""
+    import oracle.models as om
+
+    cores = len(os.sched_getaffinity(0))
+    rng = np.random.default_rng(0)
+    L = cfg["L"]
+    if cfg["model"] != "resconv":
+        return None
+    N = L * L
+    spec = om.ResConvSpec(cfg["nblocks"], cfg["channels"], (L, L))
+    theta = spec.he_normal(rng).astype(np.float64)
+    batch = 16
+    s = (2 * rng.integers(0, 2, size=(batch, N)) - 1).astype(np.int8)
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        om.resconv_forward(spec, theta, s)
+        n += batch
+    dt = time.perf_counter() - t0
+    fwd_per_s = n / dt
+    ns = cfg["ns"]
+    fwd_per_step = ns * (2 * N) + ns * 550  # sweep proposals + ~550 connected configurations per sample
+    return dict(value=fwd_per_s / fwd_per_step, unit="VMC steps/s (extrapolated)", cores=cores, kind="port",
+                sample=f"{n} ResConv forwards of the float64 numpy oracle in {dt:.1f} s on {cores} cores "
+                       f"({fwd_per_s:.1f} forwards/s), scaled by {fwd_per_step} forwards per VMC step")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    import torch
+
+    import quantax_amd as qtx
+    from quantax_amd import _lib
+
+    _lib.device()
+    cfg = CONFIGS[args.config]
+    modes = ["mixed", "plain"] if args.mode == "both" else [args.mode]
+    res = {}
+    for m in modes:
+        print(f"[bench] building {args.config} {m}", file=sys.stderr, flush=True)
+        res[m] = run_mode(cfg, m, args, torch, qtx)
+        r = res[m]
+        print(f"[bench] {m}: {r['ms_per_step']:.1f} ms/step, stages {r['stages']}", file=sys.stderr, flush=True)
+    main_mode = modes[0]
+    r = res[main_mode]
+    N = cfg["L"] ** 2 if cfg["model"] != "rbm1d" else cfg["L"]
+    sweep_steps = args.sweep_steps or 2 * N
+    roof = None
+    if cfg["model"] == "resconv":
+        fwd = conv_flops_per_forward(cfg["nblocks"], cfg["channels"], N)
+        flops = cfg["ns"] * sweep_steps * fwd  # one Metropolis launch = one sweep
+        t = r["stages"]["sweep"] / 1e3
+        ach = flops / t / 1e12
+        peak = MFMA_PEAK["bf16"] if main_mode == "mixed" else MFMA_PEAK["fp32"]
+        roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(ach / peak, 4), "traffic": None,
+                "kernel": "rc_metropolis_kernel", "flops_per_launch": flops}
+    out = {
+        "metric": "VMC steps/s, 16x16 J1J2 ResConv MinSR (mixed vs FP64)" if args.config == "c3" else
+        f"VMC steps/s ({args.config})",
+        "value": round(r["steps_per_s"], 5),
+        "unit": "VMC steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_step"], 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3+fp32+fp64" if main_mode == "mixed" else "fp32+fp64",
+        "data": "synthetic (He-normal ResConv weights, Philox-seeded chains)",
+        "config": {"workload": cfg["workload"], "mode": main_mode, "nsamples": cfg["ns"],
+                   "sweep_steps": sweep_steps, "solver": args.solver, "nparams": r["nparams"]},
+        "stages_ms": {k: round(v, 2) for k, v in r["stages"].items()},
+        "energy": r["energies"][-1],
+        "acceptance": round(r["acceptance"], 4),
+        "mean_nconn": round(r["nconn"], 1),
+        "roofline": roof,
+    }
+    if "plain" in res and main_mode != "plain":
+        p = res["plain"]
+        out["plain"] = {"value": round(p["steps_per_s"], 5), "ms_per_step": round(p["ms_per_step"], 2),
+                        "stages_ms": {k: round(v, 2) for k, v in p["stages"].items()}}
+        out["mixed_over_plain"] = round(r["steps_per_s"] / p["steps_per_s"], 3)
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
