@@ -280,9 +280,9 @@ __device__ __forceinline__ void conv_item(const char* act_cfg, const RcGeom& g, 
 // Result: lp_out[b] (LDS double).  All threads of the block must call it.
 // act: two LDS activation buffers of B * N * C * 4 bytes each (ping-pong: a
 // layer reads one and writes the other, so one barrier per layer suffices).
-// red: >= 2 * B * RC_WAVES floats.  Items of wave w: w + RC_WAVES * k, k < MAXT.
-template <int C, int MODE, int B, int MAXT>
-__device__ void rc_forward(const RcGeom& g, int nb, int gelu_exact, const char* __restrict__ packed,
+// red: >= 2 * B * W floats.  Items of wave w: w + W * k, k < MAXT.
+template <int C, int MODE, int B, int MAXT, int W>
+__device__ __attribute__((always_inline)) inline void rc_forward(const RcGeom& g, int nb, int gelu_exact, const char* __restrict__ packed,
                            int64_t off_l0w, int64_t off_l0b, int64_t bias_off, int64_t off_frag, int64_t frag_bytes,
                            const int8_t* sp, char* act, float* red, double* lp_out) {
   using K = RcK<C, MODE>;
@@ -302,7 +302,7 @@ __device__ void rc_forward(const RcGeom& g, int nb, int gelu_exact, const char* 
     const float* b0 = pf + off_l0b;
 #pragma unroll
     for (int k = 0; k < MAXT; ++k) {
-      int it = wv + RC_WAVES * k;
+      int it = wv + W * k;
       if (it < nitems) {
         int b = it / g.NT, t = it % g.NT;
         int p = 16 * t + lane % 16;
@@ -366,7 +366,7 @@ __device__ void rc_forward(const RcGeom& g, int nb, int gelu_exact, const char* 
     }
 #pragma unroll
     for (int k = 0; k < MAXT; ++k) {
-      int it = wv + RC_WAVES * k;
+      int it = wv + W * k;
       if (it < nitems) {
         int b = it / g.NT, t = it % g.NT;
         int p = 16 * t + lane % 16;
@@ -431,19 +431,19 @@ __device__ void rc_forward(const RcGeom& g, int nb, int gelu_exact, const char* 
       if (lane == 0) {
 #pragma unroll
         for (int b = 0; b < B; ++b) {
-          red[(b * RC_WAVES + wv) * 2] = m_b[b];
-          red[(b * RC_WAVES + wv) * 2 + 1] = s_b[b];
+          red[(b * W + wv) * 2] = m_b[b];
+          red[(b * W + wv) * 2 + 1] = s_b[b];
         }
       }
       __syncthreads();
       if (threadIdx.x < B) {
         int b = threadIdx.x;
         float m = -INFINITY;
-        for (int w = 0; w < RC_WAVES; ++w) m = fmaxf(m, red[(b * RC_WAVES + w) * 2]);
+        for (int w = 0; w < W; ++w) m = fmaxf(m, red[(b * W + w) * 2]);
         double ssum = 0.0;
-        for (int w = 0; w < RC_WAVES; ++w) {
-          float mw = red[(b * RC_WAVES + w) * 2];
-          if (mw > -INFINITY) ssum += (double)red[(b * RC_WAVES + w) * 2 + 1] * exp((double)mw - (double)m);
+        for (int w = 0; w < W; ++w) {
+          float mw = red[(b * W + w) * 2];
+          if (mw > -INFINITY) ssum += (double)red[(b * W + w) * 2 + 1] * exp((double)mw - (double)m);
         }
         lp_out[b] = (double)m + log(ssum) - log((double)C * (double)g.N);
       }
@@ -456,6 +456,225 @@ __device__ void rc_forward(const RcGeom& g, int nb, int gelu_exact, const char* 
 
 namespace qtx {
 
+// Row-tile forward for C = 16 on L0 x 16 grids (L0 <= 16): an item is one
+// lattice row of one configuration, so the 16 lanes of a fragment are the 16
+// columns and every tap address is a wave-uniform row offset plus a per-lane
+// column constant computed once.  Same LDS layout and results as rc_forward.
+template <int MODE, int B, int MAXT, int W>
+__device__ __attribute__((always_inline)) inline void rc_forward_rows(const RcGeom& g, int nb, int gelu_exact, const char* __restrict__ packed,
+                                int64_t off_l0w, int64_t off_l0b, int64_t bias_off, int64_t off_frag,
+                                int64_t frag_bytes, const int8_t* sp, char* act, float* red, double* lp_out) {
+  constexpr int C = 16;
+  using K = RcK<C, MODE>;
+  constexpr int ROWB = 16 * 64;  // bytes of one lattice row (16 positions x 16 channels x 4 B)
+  const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+  const int gq = lane / 16, x = lane % 16;
+  const int L0 = g.L0;
+  const int nitems = B * L0;
+  const int cfg_bytes = L0 * ROWB;
+  const int buf_bytes = cfg_bytes * B;
+  const float* pf = reinterpret_cast<const float*>(packed);
+
+  // per-lane column constants (swizzle key of column c is (c >> 2) & 3)
+  int xoff[3];
+#pragma unroll
+  for (int dx = 0; dx < 3; ++dx) {
+    int xs = (x + dx + 15) & 15;
+    xoff[dx] = xs * 64 + ((gq ^ ((xs >> 2) & 3)) << 4);
+  }
+  int kdy[5], khi[5], klo[5];
+#pragma unroll
+  for (int ks = 0; ks < 5; ++ks) {
+    int tap = 2 * ks + (gq >> 1);
+    bool valid = tap < 9;
+    int dy = valid ? tap / 3 : 1, dx = valid ? tap % 3 : 1;
+    int xs = (x + dx + 15) & 15, key = (xs >> 2) & 3, jh = gq & 1;
+    kdy[ks] = dy;
+    khi[ks] = xs * 64 + ((jh ^ key) << 4);
+    klo[ks] = xs * 64 + (((2 + jh) ^ key) << 4);
+  }
+  const int key0 = (x >> 2) & 3;
+  const int st_f32 = x * 64 + ((gq ^ key0) << 4);
+  const int st_hi = x * 64 + (((gq >> 1) ^ key0) << 4) + (gq & 1) * 8;
+  const int st_lo = x * 64 + (((2 + (gq >> 1)) ^ key0) << 4) + (gq & 1) * 8;
+
+  auto store4 = [&](char* row_base, v4f v) {
+    if constexpr (MODE == QTX_CONV_FP32) {
+      *reinterpret_cast<v4f*>(row_base + st_f32) = v;
+    } else {
+      v4bf hi, lo;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        __bf16 h = (__bf16)v[q];
+        hi[q] = h;
+        lo[q] = (__bf16)(v[q] - (float)h);
+      }
+      *reinterpret_cast<v4bf*>(row_base + st_hi) = hi;
+      *reinterpret_cast<v4bf*>(row_base + st_lo) = lo;
+    }
+  };
+  auto wrap = [&](int r) { return r < 0 ? r + L0 : (r >= L0 ? r - L0 : r); };
+
+  v4f res[MAXT];
+
+  // ---- layer 0: conv (Cin = 1) of s / sqrt(2) on VALU, GELU, into buffer 0
+  {
+    const float* W0 = pf + off_l0w;
+    const float* b0 = pf + off_l0b;
+#pragma unroll
+    for (int k = 0; k < MAXT; ++k) {
+      int it = wv + W * k;
+      if (it < nitems) {
+        int b = it / L0, y = it % L0;
+        const int8_t* s = sp + b * 16 * L0;
+        float sv[9];
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap)
+          sv[tap] = (float)s[wrap(y + tap / 3 - 1) * 16 + ((x + tap % 3 + 15) & 15)] * 0.70710678118654752f;
+        float spin = (float)s[y * 16 + x];
+        v4f z;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          int co = 4 * gq + q;
+          float a = b0[co];
+#pragma unroll
+          for (int tap = 0; tap < 9; ++tap) a += W0[co * 9 + tap] * sv[tap];
+          z[q] = gelu_f(a, gelu_exact);
+        }
+        res[k] = v4f{spin, spin, spin, spin};
+        store4(act + b * cfg_bytes + y * ROWB, z);
+      }
+    }
+  }
+  __syncthreads();
+
+  int cur = 0;
+  for (int l = 1; l < 2 * nb; ++l) {
+    float wf[(MODE == QTX_CONV_FP32) ? K::KS : 1];
+    v8bf wh[(MODE == QTX_CONV_FP32) ? 1 : K::KS];
+    v8bf wl[(MODE == QTX_CONV_FP32) ? 1 : K::KS];
+    const char* fr = packed + off_frag + (int64_t)l * frag_bytes;
+    if constexpr (MODE == QTX_CONV_FP32) {
+#pragma unroll
+      for (int i = 0; i < K::KS; ++i) wf[i] = reinterpret_cast<const float*>(fr)[i * WAVE + lane];
+    } else {
+#pragma unroll
+      for (int i = 0; i < K::KS; ++i) {
+        const v8bf* src = reinterpret_cast<const v8bf*>(fr) + ((int64_t)i * WAVE + lane) * 2;
+        wh[i] = src[0];
+        wl[i] = src[1];
+      }
+    }
+    const float* bias = pf + bias_off + (int64_t)l * C;
+    const v4f bias4 = v4f{bias[4 * gq], bias[4 * gq + 1], bias[4 * gq + 2], bias[4 * gq + 3]};
+    const char* in = act + cur * buf_bytes;
+    char* out = act + (cur ^ 1) * buf_bytes;
+    const bool is_conv2 = (l % 2) == 1;
+    const int blk = l / 2;
+    const bool last = (l == 2 * nb - 1);
+    const float scale = last ? rsqrtf((float)(nb + 1)) : (is_conv2 ? rsqrtf((float)(blk + 2)) : 1.0f);
+    float m_b[B], s_b[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      m_b[b] = -INFINITY;
+      s_b[b] = 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < MAXT; ++k) {
+      int it = wv + W * k;
+      if (it < nitems) {
+        int b = it / L0, y = it % L0;
+        const char* base = in + b * cfg_bytes;
+        v4f acc = bias4;
+        if constexpr (MODE == QTX_CONV_FP32) {
+#pragma unroll
+          for (int dy = 0; dy < 3; ++dy) {
+            const char* rb = base + wrap(y + dy - 1) * ROWB;
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+              v4f bv = *reinterpret_cast<const v4f*>(rb + xoff[dx]);
+              const int tap = 3 * dy + dx;
+#pragma unroll
+              for (int jp = 0; jp < 4; ++jp)
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(wf[tap * 4 + jp], bv[jp], acc, 0, 0, 0);
+            }
+          }
+        } else {
+#pragma unroll
+          for (int ks = 0; ks < 5; ++ks) {
+            const char* rb = base + wrap(y + kdy[ks] - 1) * ROWB;
+            v8bf bh = *reinterpret_cast<const v8bf*>(rb + khi[ks]);
+            v8bf bl = *reinterpret_cast<const v8bf*>(rb + klo[ks]);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[ks], bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[ks], bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[ks], bh, acc, 0, 0, 0);
+          }
+        }
+        if (!last) {
+          v4f o;
+          if (is_conv2) {
+            v4f xr = acc + res[k];
+            res[k] = xr;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) o[q] = gelu_f(xr[q] * scale, gelu_exact);
+          } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) o[q] = gelu_f(acc[q], gelu_exact);
+          }
+          store4(out + b * cfg_bytes + y * ROWB, o);
+          // keep one item's fragments live at a time (bounds VGPR use)
+          __builtin_amdgcn_sched_barrier(0);
+        } else {
+          float hv[4], m = -INFINITY;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            hv[q] = (acc[q] + res[k][q]) * scale;
+            m = fmaxf(m, hv[q]);
+          }
+          m = wave_max(m);
+          float sacc = 0.0f;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) sacc += __expf(hv[q] - m);
+          sacc = wave_sum(sacc);
+#pragma unroll
+          for (int bb = 0; bb < B; ++bb) {
+            if (bb == b) {
+              float mm = fmaxf(m_b[bb], m);
+              s_b[bb] = s_b[bb] * __expf(m_b[bb] - mm) + sacc * __expf(m - mm);
+              m_b[bb] = mm;
+            }
+          }
+        }
+      }
+    }
+    if (!last) {
+      __syncthreads();
+      cur ^= 1;
+    } else {
+      if (lane == 0) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+          red[(b * W + wv) * 2] = m_b[b];
+          red[(b * W + wv) * 2 + 1] = s_b[b];
+        }
+      }
+      __syncthreads();
+      if (threadIdx.x < B) {
+        int b = threadIdx.x;
+        float m = -INFINITY;
+        for (int w = 0; w < W; ++w) m = fmaxf(m, red[(b * W + w) * 2]);
+        double ssum = 0.0;
+        for (int w = 0; w < W; ++w) {
+          float mw = red[(b * W + w) * 2];
+          if (mw > -INFINITY) ssum += (double)red[(b * W + w) * 2 + 1] * exp((double)mw - (double)m);
+        }
+        lp_out[b] = (double)m + log(ssum) - log((double)C * (double)g.N);
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // ---------------------------------------------------------------- kernels
 struct RcArgs {
   RcGeom g;
@@ -464,20 +683,26 @@ struct RcArgs {
   int64_t off_l0w, off_l0b, bias_off, off_frag, frag_bytes;
 };
 
-#define RC_FWD(SP, LP) \
-  rc_forward<C, MODE, B, MAXT>(a.g, a.nb, a.gelu_exact, a.packed, a.off_l0w, a.off_l0b, a.bias_off, a.off_frag, \
-                               a.frag_bytes, SP, act, red, LP)
+#define RC_FWD(SP, LP)                                                                                        \
+  do {                                                                                                        \
+    if constexpr (ROWS)                                                                                       \
+      rc_forward_rows<MODE, B, MAXT, W>(a.g, a.nb, a.gelu_exact, a.packed, a.off_l0w, a.off_l0b, a.bias_off,   \
+                                        a.off_frag, a.frag_bytes, SP, act, red, LP);                          \
+    else                                                                                                      \
+      rc_forward<C, MODE, B, MAXT, W>(a.g, a.nb, a.gelu_exact, a.packed, a.off_l0w, a.off_l0b, a.bias_off,     \
+                                      a.off_frag, a.frag_bytes, SP, act, red, LP);                            \
+  } while (0)
 
-template <int C, int MODE, int B, int NMAX, int MAXT>
-__global__ __launch_bounds__(RC_THREADS) void rc_logpsi_kernel(RcArgs a, const int8_t* __restrict__ s, int64_t ns,
+template <int C, int MODE, int B, int NMAX, int MAXT, int W, int ROWS>
+__global__ __launch_bounds__(W * 64, (W == 8 ? 2 : 1)) void rc_logpsi_kernel(RcArgs a, const int8_t* __restrict__ s, int64_t ns,
                                                                double* __restrict__ logabs) {
   __shared__ __attribute__((aligned(16))) char act[2 * B * NMAX * C * 4];
   __shared__ int8_t sp[B * NMAX];
-  __shared__ float red[2 * B * RC_WAVES];
+  __shared__ float red[2 * B * W];
   __shared__ double lp[B];
   const int N = a.g.N;
   int64_t c0 = (int64_t)blockIdx.x * B;
-  for (int i = threadIdx.x; i < B * N; i += RC_THREADS) {
+  for (int i = threadIdx.x; i < B * N; i += (W * WAVE)) {
     int b = i / N;
     sp[i] = (c0 + b < ns) ? s[(c0 + b) * N + (i % N)] : (int8_t)1;
   }
@@ -486,15 +711,15 @@ __global__ __launch_bounds__(RC_THREADS) void rc_logpsi_kernel(RcArgs a, const i
   if (threadIdx.x < B && c0 + threadIdx.x < ns) logabs[c0 + threadIdx.x] = lp[threadIdx.x];
 }
 
-template <int C, int MODE, int B, int NMAX, int MAXT>
-__global__ __launch_bounds__(RC_THREADS) void rc_metropolis_kernel(
+template <int C, int MODE, int B, int NMAX, int MAXT, int W, int ROWS>
+__global__ __launch_bounds__(W * 64, (W == 8 ? 2 : 1)) void rc_metropolis_kernel(
     RcArgs a, int kind, const int32_t* __restrict__ nbr, int width, int hopping, const uint32_t* __restrict__ words,
     int nsteps, int64_t ns, double reweight, double tie_tol, int8_t* __restrict__ s, double* __restrict__ logabs,
     int32_t* __restrict__ n_accept, int32_t* __restrict__ n_near) {
   __shared__ __attribute__((aligned(16))) char act[2 * B * NMAX * C * 4];
   __shared__ int8_t cur[B * NMAX];
   __shared__ int8_t prop[B * NMAX];
-  __shared__ float red[2 * B * RC_WAVES];
+  __shared__ float red[2 * B * W];
   __shared__ double lp_new[B];
   __shared__ double lp_cur[B];
   __shared__ int mv_a[B], mv_b[B], mv_upd[B], nhop[B], nacc[B], nnear[B];
@@ -502,7 +727,7 @@ __global__ __launch_bounds__(RC_THREADS) void rc_metropolis_kernel(
   const int N = a.g.N;
   const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
   int64_t c0 = (int64_t)blockIdx.x * B;
-  for (int i = threadIdx.x; i < B * N; i += RC_THREADS) {
+  for (int i = threadIdx.x; i < B * N; i += (W * WAVE)) {
     int b = i / N;
     cur[i] = (c0 + b < ns) ? s[(c0 + b) * N + (i % N)] : (int8_t)1;
   }
@@ -513,7 +738,7 @@ __global__ __launch_bounds__(RC_THREADS) void rc_metropolis_kernel(
     nnear[b] = 0;
   }
   __syncthreads();
-  for (int b = wv; b < B; b += RC_WAVES) {
+  for (int b = wv; b < B; b += W) {
     int cnt = 0;
     for (int i = lane; i < N; i += WAVE) cnt += (cur[b * N + i] == hopping);
     cnt = wave_sum(cnt);
@@ -522,7 +747,7 @@ __global__ __launch_bounds__(RC_THREADS) void rc_metropolis_kernel(
   __syncthreads();
   for (int t = 0; t < nsteps; ++t) {
     // proposals: one wave per chain
-    for (int b = wv; b < B; b += RC_WAVES) {
+    for (int b = wv; b < B; b += W) {
       int64_t c = c0 + b;
       int fa = -1, fb = -1;
       bool upd = false;
@@ -547,7 +772,7 @@ __global__ __launch_bounds__(RC_THREADS) void rc_metropolis_kernel(
     __syncthreads();
     RC_FWD(prop, lp_new);
     // accept / reject (rc_forward ends with a barrier)
-    for (int b = wv; b < B; b += RC_WAVES) {
+    for (int b = wv; b < B; b += W) {
       if (!mv_upd[b]) continue;
       double d = reweight * (lp_new[b] - lp_cur[b]);
       double thr = mv_thr[b];
@@ -566,7 +791,7 @@ __global__ __launch_bounds__(RC_THREADS) void rc_metropolis_kernel(
     }
     __syncthreads();
   }
-  for (int i = threadIdx.x; i < B * N; i += RC_THREADS) {
+  for (int i = threadIdx.x; i < B * N; i += (W * WAVE)) {
     int b = i / N;
     if (c0 + b < ns) s[(c0 + b) * N + (i % N)] = cur[i];
   }
@@ -580,14 +805,14 @@ __global__ __launch_bounds__(RC_THREADS) void rc_metropolis_kernel(
 
 constexpr int RC_MAXCAND = 4096;
 
-template <int C, int MODE, int B, int NMAX, int MAXT>
-__global__ __launch_bounds__(RC_THREADS) void rc_eloc_kernel(RcArgs a, qtx_ham H, const int8_t* __restrict__ s,
+template <int C, int MODE, int B, int NMAX, int MAXT, int W, int ROWS>
+__global__ __launch_bounds__(W * 64, (W == 8 ? 2 : 1)) void rc_eloc_kernel(RcArgs a, qtx_ham H, const int8_t* __restrict__ s,
                                                              const double* __restrict__ logabs, int64_t ns,
                                                              double* __restrict__ eloc, int32_t* __restrict__ nconn) {
   __shared__ __attribute__((aligned(16))) char act[2 * B * NMAX * C * 4];
   __shared__ int8_t s0[NMAX];
   __shared__ int8_t prop[B * NMAX];
-  __shared__ float red[2 * B * RC_WAVES];
+  __shared__ float red[2 * B * W];
   __shared__ double lp[B];
   __shared__ int16_t vlist[RC_MAXCAND];
   __shared__ int nvalid_s;
@@ -595,7 +820,7 @@ __global__ __launch_bounds__(RC_THREADS) void rc_eloc_kernel(RcArgs a, qtx_ham H
   const int N = a.g.N;
   const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
   const int64_t n = blockIdx.x;
-  for (int i = threadIdx.x; i < N; i += RC_THREADS) s0[i] = s[n * N + i];
+  for (int i = threadIdx.x; i < N; i += (W * WAVE)) s0[i] = s[n * N + i];
   __syncthreads();
   if (wv == 0) {
     // diagonal part and the ordered list of valid candidates (ballot compaction)
@@ -619,7 +844,7 @@ __global__ __launch_bounds__(RC_THREADS) void rc_eloc_kernel(RcArgs a, qtx_ham H
   const double lp0 = logabs[n];
   double e = 0.0;  // accumulated by thread 0 in candidate order
   for (int start = 0; start < nvalid; start += B) {
-    for (int i = threadIdx.x; i < B * N; i += RC_THREADS) {
+    for (int i = threadIdx.x; i < B * N; i += (W * WAVE)) {
       int b = i / N, j = i % N;
       int8_t v = s0[j];
       if (start + b < nvalid) {
@@ -877,30 +1102,38 @@ static RcArgs rc_args(const RcLayout& L, const void* packed) {
 // (C = 16: N <= 256 -> 2 configurations per workgroup, N <= 512 -> 1; C = 32: N <= 400 -> 1).
 #define RC_DISPATCH(KERNEL, NS_PER_BLOCK_OUT, GRID_FROM_B, ...)                                        \
   do {                                                                                                 \
-    if (L.C == 16 && L.N <= 256) {                                                                     \
+    if (L.C == 16 && L.L1 == 16 && L.L0 <= 16) {                                                       \
       constexpr int B_ = 2;                                                                            \
       NS_PER_BLOCK_OUT = B_;                                                                           \
       dim3 grid_ = GRID_FROM_B;                                                                        \
       if (L.mode == QTX_CONV_FP32)                                                                     \
-        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_FP32, 2, 256, 8>), grid_, dim3(RC_THREADS), 0, st, __VA_ARGS__); \
+        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_FP32, 2, 256, 4, 8, 1>), grid_, dim3(512), 0, st, __VA_ARGS__); \
       else                                                                                             \
-        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_BF16X3, 2, 256, 8>), grid_, dim3(RC_THREADS), 0, st, __VA_ARGS__); \
+        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_BF16X3, 2, 256, 4, 8, 1>), grid_, dim3(512), 0, st, __VA_ARGS__); \
+    } else if (L.C == 16 && L.N <= 256) {                                                              \
+      constexpr int B_ = 2;                                                                            \
+      NS_PER_BLOCK_OUT = B_;                                                                           \
+      dim3 grid_ = GRID_FROM_B;                                                                        \
+      if (L.mode == QTX_CONV_FP32)                                                                     \
+        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_FP32, 2, 256, 8, 4, 0>), grid_, dim3(256), 0, st, __VA_ARGS__); \
+      else                                                                                             \
+        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_BF16X3, 2, 256, 8, 4, 0>), grid_, dim3(256), 0, st, __VA_ARGS__); \
     } else if (L.C == 16) {                                                                            \
       constexpr int B_ = 1;                                                                            \
       NS_PER_BLOCK_OUT = B_;                                                                           \
       dim3 grid_ = GRID_FROM_B;                                                                        \
       if (L.mode == QTX_CONV_FP32)                                                                     \
-        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_FP32, 1, 512, 8>), grid_, dim3(RC_THREADS), 0, st, __VA_ARGS__); \
+        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_FP32, 1, 512, 8, 4, 0>), grid_, dim3(256), 0, st, __VA_ARGS__); \
       else                                                                                             \
-        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_BF16X3, 1, 512, 8>), grid_, dim3(RC_THREADS), 0, st, __VA_ARGS__); \
+        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_BF16X3, 1, 512, 8, 4, 0>), grid_, dim3(256), 0, st, __VA_ARGS__); \
     } else {                                                                                           \
       constexpr int B_ = 1;                                                                            \
       NS_PER_BLOCK_OUT = B_;                                                                           \
       dim3 grid_ = GRID_FROM_B;                                                                        \
       if (L.mode == QTX_CONV_FP32)                                                                     \
-        hipLaunchKernelGGL((KERNEL<32, QTX_CONV_FP32, 1, 400, 7>), grid_, dim3(RC_THREADS), 0, st, __VA_ARGS__); \
+        hipLaunchKernelGGL((KERNEL<32, QTX_CONV_FP32, 1, 400, 7, 4, 0>), grid_, dim3(256), 0, st, __VA_ARGS__); \
       else                                                                                             \
-        hipLaunchKernelGGL((KERNEL<32, QTX_CONV_BF16X3, 1, 400, 7>), grid_, dim3(RC_THREADS), 0, st, __VA_ARGS__); \
+        hipLaunchKernelGGL((KERNEL<32, QTX_CONV_BF16X3, 1, 400, 7, 4, 0>), grid_, dim3(256), 0, st, __VA_ARGS__); \
     }                                                                                                  \
   } while (0)
 

@@ -572,6 +572,27 @@ int qtx_minsr_solve(void* stream, int o_dtype, const void* O, const double* mean
                                step, w);
 }
 
+int qtx_gram(void* stream, int o_dtype, const void* O, const double* mean, const double* r, int64_t ns,
+             int64_t np, int gram, double* T) {
+  QTX_REQUIRE(o_dtype == QTX_F32 || o_dtype == QTX_F64, "o_dtype must be QTX_F32 or QTX_F64");
+  QTX_REQUIRE(ns > 0 && np > 0 && O && mean && T, "bad arguments");
+  QTX_REQUIRE(gram == QTX_GRAM_FP64 || gram == QTX_GRAM_FP32 || gram == QTX_GRAM_BF16X3, "unknown gram mode");
+  hipStream_t st = as_stream(stream);
+  if (ns < np) {
+    if (o_dtype == QTX_F32) return launch_gram_nt<float>(st, gram, (const float*)O, mean, r, ns, np, T);
+    return launch_gram_nt<double>(st, gram, (const double*)O, mean, r, ns, np, T);
+  }
+  dim3 g(nblk(np, 16), nblk(np, 16));
+  if (o_dtype == QTX_F32)
+    hipLaunchKernelGGL(gram_tn_kernel<float>, g, dim3(256), 0, st, (const float*)O, mean, r, ns, np,
+                       1.0 / (double)ns, T);
+  else
+    hipLaunchKernelGGL(gram_tn_kernel<double>, g, dim3(256), 0, st, (const double*)O, mean, r, ns, np,
+                       1.0 / (double)ns, T);
+  QTX_LAUNCH_CHECK();
+  return QTX_OK;
+}
+
 int qtx_apply_update(void* stream, int dtype, void* theta, const double* step, double lr, int64_t np,
                      int32_t* applied) {
   QTX_REQUIRE(dtype == QTX_F32 || dtype == QTX_F64, "dtype must be QTX_F32 or QTX_F64");

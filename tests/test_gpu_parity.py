@@ -316,3 +316,76 @@ def test_apply_update_skips_nonfinite():
     with pytest.warns(UserWarning):
         assert not state.update(torch.from_numpy(bad))
     assert np.array_equal(state.get_params_flatten().cpu().numpy(), th1)
+
+
+# ------------------------------------------------------------------ row-tile kernels (16-wide lattices)
+def _resconv_rect(L0, L1, nb, C, precision, seed=5, nup=None):
+    lat = qtx.sites.Grid([L0, L1], Nparticles=None if nup is None else (nup, L0 * L1 - nup))
+    model = qtx.model.ResConv(nb, C, precision=precision, seed=seed)
+    th = model.theta.cpu().numpy().astype(np.float64)
+    r = np.random.default_rng(seed + 1)
+    spec = om.ResConvSpec(nb, C, (L0, L1))
+    for (cin, cout, hb), (w, bo) in zip(spec.layers, spec.offsets):
+        if hb:
+            th[bo: bo + cout] = 0.1 * r.standard_normal(cout)
+    model.set_params(th.astype(np.float32))
+    return lat, model, spec, model.theta.cpu().numpy().astype(np.float64)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-5), ("bf16x3", 1e-4)])
+@pytest.mark.parametrize("L0", [16, 12, 5])
+def test_resconv_rows_logpsi(precision, tol, L0, rng):
+    lat, model, spec, theta = _resconv_rect(L0, 16, 3, 16, precision)
+    s = rand_spins(rng, 7, L0 * 16)
+    la = model.logpsi(to_dev(s)).cpu().numpy()
+    assert max_rel(la, om.resconv_forward(spec, theta, s)[1]) < tol
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-5), ("bf16x3", 1e-4)])
+def test_resconv_rows_eloc_j1j2_16x16(precision, tol, rng):
+    lat, model, spec, theta = _resconv_rect(16, 16, 2, 16, precision, nup=128)
+    H = qtx.operator.Heisenberg(J=[1, 0.5], n_neighbor=[1, 2], msr=True)
+    Ho = oop.heisenberg([16, 16], [1, 0.5], [1, 2], True)
+    s = rand_spins(rng, 2, 256, nup=128)
+    la = model.logpsi(to_dev(s))
+    e, nconn = model.eloc(H, to_dev(s), la)
+    ref = oop.oloc(s, lambda x: om.resconv_forward(spec, theta, x), Ho)
+    assert max_rel(e.cpu().numpy(), ref) < tol
+
+
+def test_resconv_rows_metropolis_16x16(rng):
+    lat, model, spec, theta = _resconv_rect(16, 16, 2, 16, "fp32", nup=128)
+    ns, nsteps = 6, 8
+    s0 = rand_spins(rng, ns, 256, nup=128)
+    words = philox.sweep_words(21, nsteps, ns)
+    nbr_np = lat.neighbor_table(1)
+    s = to_dev(s0)
+    la = model.logpsi(s)
+    nacc = torch.zeros(ns, dtype=torch.int32, device=DEV)
+    nnear = torch.zeros(ns, dtype=torch.int32, device=DEV)
+    tie = 1e-4
+    model.metropolis(_lib.QTX_EXCHANGE, to_dev(nbr_np), nbr_np.shape[1], 1, to_dev(words.view(np.int32)), nsteps, s,
+                     la, nacc, nnear, 2.0, tie)
+    s_ref, la_ref, nacc_ref, near_ref = osm.metropolis(lambda x: om.resconv_forward(spec, theta, x), s0, words,
+                                                       osm.EXCHANGE, nbr_np, 1, 2.0, tie)
+    clean = (near_ref == 0) & (nnear.cpu().numpy() == 0)
+    assert clean.sum() >= ns - 1
+    assert np.array_equal(s.cpu().numpy()[clean], s_ref[clean])
+    assert max_rel(la.cpu().numpy()[clean], la_ref[clean]) < 1e-5
+
+
+@pytest.mark.parametrize("gram,tol", [("fp64", 1e-12), ("fp32", 1e-5), ("bf16x3", 1e-5)])
+def test_gram_entry_point(gram, tol, rng):
+    ns, np_ = 300, 1000
+    O = (rng.standard_normal((ns, np_)) * 0.2 + 0.05).astype(np.float32)
+    Od = to_dev(O)
+    mean = torch.empty(np_, dtype=torch.float64, device=DEV)
+    _lib.call("qtx_colmean", _lib.stream(), _lib.QTX_F32, Od.data_ptr(), ns, np_, mean.data_ptr())
+    T = torch.empty((ns, ns), dtype=torch.float64, device=DEV)
+    g = {"fp64": 0, "fp32": 1, "bf16x3": 2}[gram]
+    _lib.call("qtx_gram", _lib.stream(), _lib.QTX_F32, Od.data_ptr(), mean.data_ptr(), None, ns, np_, g, T.data_ptr())
+    Obar, _ = osr.get_Obar(O.astype(np.float64))
+    ref = Obar @ Obar.T
+    Tn = T.cpu().numpy()
+    assert np.array_equal(Tn, Tn.T)
+    assert np.max(np.abs(Tn - ref)) / np.max(np.abs(ref)) < tol
