@@ -40,6 +40,8 @@ def parse():
     p.add_argument("--solver", default="eigh", choices=["eigh", "chol"])
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--calib-copy-mb", type=int, default=0,
+                   help="run a device-to-device copy of this many MiB first (HBM counter calibration)")
     return p.parse_args()
 
 
@@ -186,6 +188,14 @@ def main():
     from quantax_amd import _lib
 
     _lib.device()
+    if args.calib_copy_mb > 0:
+        n = args.calib_copy_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copy of {args.calib_copy_mb} MiB done", file=sys.stderr, flush=True)
+        del src, dst
     cfg = CONFIGS[args.config]
     modes = ["mixed", "plain"] if args.mode == "both" else [args.mode]
     res = {}

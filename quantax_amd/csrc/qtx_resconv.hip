@@ -34,6 +34,7 @@ constexpr int RC_MAXN = 1024;
 // ---------------------------------------------------------------- layout
 struct RcLayout {
   int nb, C, L0, L1, N, mode, gelu_exact;
+  int win;                   // 16 x 16, C = 16: sliding-row-window forward and its K-step order
   int64_t nparams;
   int64_t off_l0w, off_l0b;  // first conv (Cin = 1): W (C, 9), b (C) in floats
   int64_t off_frag;          // start of per-layer fragment blocks (bytes)
@@ -86,6 +87,7 @@ inline RcLayout rc_layout(const qtx_resconv_desc* d) {
   L.N = d->L0 * d->L1;
   L.mode = d->precision;
   L.gelu_exact = d->gelu_exact;
+  L.win = (L.C == 16 && L.L0 == 16 && L.L1 == 16) ? 1 : 0;
   L.nparams = rc_nparams(L.nb, L.C);
   int64_t off = L.nparams * 4;  // raw theta copy first
   off = (off + 255) & ~(int64_t)255;
@@ -99,7 +101,7 @@ inline RcLayout rc_layout(const qtx_resconv_desc* d) {
   off = (off + 255) & ~(int64_t)255;
   L.off_frag = off;
   int ct = L.C / 16;
-  int ks = (L.mode == QTX_CONV_FP32) ? 9 * L.C / 4 : (L.C == 16 ? 5 : 9);
+  int ks = (L.mode == QTX_CONV_FP32) ? 9 * L.C / 4 : (L.C == 16 ? (L.win ? 6 : 5) : 9);
   int words = (L.mode == QTX_CONV_FP32) ? 1 : 8;
   L.frag_bytes = (int64_t)ct * ks * WAVE * words * 4;
   off += L.frag_bytes * 2 * L.nb;  // slot per layer index (slot 0 unused)
@@ -109,18 +111,20 @@ inline RcLayout rc_layout(const qtx_resconv_desc* d) {
 
 // ---------------------------------------------------------------- packing
 // one thread per (layer, ct, ks, lane) fragment for layers 1 .. 2nb-1
-template <int C, int MODE>
+template <int C, int MODE, int WIN>
 __global__ void rc_pack_frag_kernel(const float* __restrict__ theta, int nb, char* __restrict__ packed,
                                     int64_t off_frag, int64_t frag_bytes) {
-  using K = RcK<C, MODE>;
+  using K0 = RcK<C, MODE>;
+  constexpr int KCT = K0::CT;
+  constexpr int KKS = (WIN && MODE == QTX_CONV_BF16X3) ? 6 : K0::KS;
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t per_layer = (int64_t)K::CT * K::KS * WAVE;
+  int64_t per_layer = (int64_t)KCT * KKS * WAVE;
   int64_t total = per_layer * (2 * nb - 1);
   if (idx >= total) return;
   int l = 1 + (int)(idx / per_layer);
   int64_t rem = idx % per_layer;
-  int ct = (int)(rem / (K::KS * WAVE));
-  int ks = (int)((rem / WAVE) % K::KS);
+  int ct = (int)(rem / (KKS * WAVE));
+  int ks = (int)((rem / WAVE) % KKS);
   int lane = (int)(rem % WAVE);
   int64_t w0, b0;
   rc_theta_offsets(nb, C, l, w0, b0);
@@ -131,17 +135,22 @@ __global__ void rc_pack_frag_kernel(const float* __restrict__ theta, int nb, cha
     // ks = (tap, h, j'): ci = 16 h + 4 g + j'
     int tap = ks / (C / 4), rr = ks % (C / 4), h = rr / 4, jp = rr % 4;
     float v = W(16 * h + 4 * g + jp, tap);
-    reinterpret_cast<float*>(dst)[((int64_t)ct * K::KS + ks) * WAVE + lane] = v;
+    reinterpret_cast<float*>(dst)[((int64_t)ct * KKS + ks) * WAVE + lane] = v;
   } else {
     int tap, c0;
-    if (C == 16) {
+    if (WIN) {
+      // ks = 2 dy + pr; pr 0: taps (dy, 0), (dy, 1); pr 1: tap (dy, 2) and a zero pad
+      int dy = ks / 2, pr = ks % 2;
+      tap = (pr == 0) ? 3 * dy + (g >> 1) : ((g >> 1) == 0 ? 3 * dy + 2 : 9);
+      c0 = 8 * (g & 1);
+    } else if (C == 16) {
       tap = 2 * ks + (g >> 1);
       c0 = 8 * (g & 1);
     } else {
       tap = ks;
       c0 = 8 * g;
     }
-    __bf16* o = reinterpret_cast<__bf16*>(dst) + (((int64_t)ct * K::KS + ks) * WAVE + lane) * 16;
+    __bf16* o = reinterpret_cast<__bf16*>(dst) + (((int64_t)ct * KKS + ks) * WAVE + lane) * 16;
     for (int q = 0; q < 8; ++q) {
       float v = (tap < 9) ? W(c0 + q, tap) : 0.0f;
       __bf16 hi = (__bf16)v;
@@ -170,11 +179,14 @@ __global__ void rc_pack_misc_kernel(const float* __restrict__ theta, int nb, int
 }
 
 // ---------------------------------------------------------------- math
+// tanh form: 0.5 x (1 + tanh(k (x + 0.044715 x^3))) = x / (1 + 2^(x (A + B x^2)))
+// with A = -2 k log2(e), B = -2 k 0.044715 log2(e), k = sqrt(2 / pi); native exp2 and rcp
 __device__ __forceinline__ float gelu_f(float x, int exact) {
   if (exact) return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f));
-  float u = 0.79788456080286536f * (x + 0.044715f * x * x * x);
-  float t = 1.0f - 2.0f / (1.0f + __expf(2.0f * u));
-  return 0.5f * x * (1.0f + t);
+  constexpr float A = -2.0f * 0.79788456080286536f * 1.4426950408889634f;
+  constexpr float Bc = A * 0.044715f;
+  float e = __builtin_amdgcn_exp2f(x * fmaf(x * x, Bc, A));
+  return x * __builtin_amdgcn_rcpf(1.0f + e);
 }
 
 __device__ __forceinline__ float gelu_grad_f(float x, int exact) {
@@ -675,6 +687,225 @@ __device__ __attribute__((always_inline)) inline void rc_forward_rows(const RcGe
   }
 }
 
+// Sliding-row-window forward for C = 16 on a 16 x 16 grid.  Workgroup: 8
+// waves, 2 configurations; wave w owns rows 4 (w % 4) .. 4 (w % 4) + 3 of
+// configuration w / 4 and walks them in order keeping the B fragments of
+// input rows y - 1, y, y + 1 in registers: every fragment depends only on its
+// input row (split-bf16 K-steps 2 dy + pr: pr 0 = taps (dy, 0) + (dy, 1),
+// pr 1 = tap (dy, 2) + a zero pad; FP32 K-steps = taps), so each row is read
+// from LDS once per wave and layer instead of three times.
+template <int MODE>
+__device__ __attribute__((always_inline)) inline void rc_forward_win(
+    const RcGeom& g, int nb, int gelu_exact, const char* __restrict__ packed, int64_t off_l0w, int64_t off_l0b,
+    int64_t bias_off, int64_t off_frag, int64_t frag_bytes, const int8_t* sp, char* act, float* red,
+    double* lp_out) {
+  constexpr int C = 16, B = 2, W = 8, L0 = 16, RPW = 4;
+  constexpr int ROWB = 16 * 64;
+  constexpr int CFGB = L0 * ROWB;
+  constexpr int BUFB = CFGB * B;
+  constexpr int KSW = (MODE == QTX_CONV_FP32) ? 36 : 6;
+  const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+  const int gq = lane / 16, x = lane % 16;
+  const int b = wv / 4, y0 = RPW * (wv % 4);
+  const float* pf = reinterpret_cast<const float*>(packed);
+
+  int xoff[3];
+#pragma unroll
+  for (int dx = 0; dx < 3; ++dx) {
+    int xs = (x + dx + 15) & 15;
+    xoff[dx] = xs * 64 + ((gq ^ ((xs >> 2) & 3)) << 4);
+  }
+  int khi[2], klo[2];
+#pragma unroll
+  for (int pr = 0; pr < 2; ++pr) {
+    int dx = (pr == 0) ? (gq >> 1) : 2;
+    int xs = (x + dx + 15) & 15, key = (xs >> 2) & 3, jh = gq & 1;
+    khi[pr] = xs * 64 + ((jh ^ key) << 4);
+    klo[pr] = xs * 64 + (((2 + jh) ^ key) << 4);
+  }
+  const int key0 = (x >> 2) & 3;
+  const int st_f32 = x * 64 + ((gq ^ key0) << 4);
+  const int st_hi = x * 64 + (((gq >> 1) ^ key0) << 4) + (gq & 1) * 8;
+  const int st_lo = x * 64 + (((2 + (gq >> 1)) ^ key0) << 4) + (gq & 1) * 8;
+  auto store4 = [&](char* row_base, v4f v) {
+    if constexpr (MODE == QTX_CONV_FP32) {
+      *reinterpret_cast<v4f*>(row_base + st_f32) = v;
+    } else {
+      v4bf hi, lo;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        __bf16 h = (__bf16)v[q];
+        hi[q] = h;
+        lo[q] = (__bf16)(v[q] - (float)h);
+      }
+      *reinterpret_cast<v4bf*>(row_base + st_hi) = hi;
+      *reinterpret_cast<v4bf*>(row_base + st_lo) = lo;
+    }
+  };
+
+  // B fragments of one input row
+  struct Frag {
+    v4f f[(MODE == QTX_CONV_FP32) ? 3 : 1];
+    v8bf h[(MODE == QTX_CONV_FP32) ? 1 : 2];
+    v8bf l[(MODE == QTX_CONV_FP32) ? 1 : 2];
+  };
+  auto load_row = [&](const char* cfg, int r) {
+    Frag F;
+    const char* rb = cfg + ((r + L0) & (L0 - 1)) * ROWB;
+    if constexpr (MODE == QTX_CONV_FP32) {
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx) F.f[dx] = *reinterpret_cast<const v4f*>(rb + xoff[dx]);
+    } else {
+#pragma unroll
+      for (int pr = 0; pr < 2; ++pr) {
+        F.h[pr] = *reinterpret_cast<const v8bf*>(rb + khi[pr]);
+        F.l[pr] = *reinterpret_cast<const v8bf*>(rb + klo[pr]);
+      }
+    }
+    return F;
+  };
+
+  v4f res[RPW];
+  // ---- layer 0 on VALU
+  {
+    const float* W0 = pf + off_l0w;
+    const float* b0 = pf + off_l0b;
+    const int8_t* s = sp + b * 256;
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+      const int y = y0 + i;
+      float sv[9];
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap)
+        sv[tap] = (float)s[((y + tap / 3 - 1 + L0) & (L0 - 1)) * 16 + ((x + tap % 3 + 15) & 15)] *
+                  0.70710678118654752f;
+      float spin = (float)s[y * 16 + x];
+      v4f z;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int co = 4 * gq + q;
+        float a = b0[co];
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) a += W0[co * 9 + tap] * sv[tap];
+        z[q] = gelu_f(a, gelu_exact);
+      }
+      res[i] = v4f{spin, spin, spin, spin};
+      store4(act + b * CFGB + y * ROWB, z);
+    }
+  }
+  __syncthreads();
+
+  int cur = 0;
+  for (int l = 1; l < 2 * nb; ++l) {
+    float wf[(MODE == QTX_CONV_FP32) ? KSW : 1];
+    v8bf wh[(MODE == QTX_CONV_FP32) ? 1 : KSW];
+    v8bf wl[(MODE == QTX_CONV_FP32) ? 1 : KSW];
+    const char* fr = packed + off_frag + (int64_t)l * frag_bytes;
+    if constexpr (MODE == QTX_CONV_FP32) {
+#pragma unroll
+      for (int i = 0; i < KSW; ++i) wf[i] = reinterpret_cast<const float*>(fr)[i * WAVE + lane];
+    } else {
+#pragma unroll
+      for (int i = 0; i < KSW; ++i) {
+        const v8bf* src = reinterpret_cast<const v8bf*>(fr) + ((int64_t)i * WAVE + lane) * 2;
+        wh[i] = src[0];
+        wl[i] = src[1];
+      }
+    }
+    const float* bias = pf + bias_off + (int64_t)l * C;
+    const v4f bias4 = v4f{bias[4 * gq], bias[4 * gq + 1], bias[4 * gq + 2], bias[4 * gq + 3]};
+    const char* in = act + cur * BUFB + b * CFGB;
+    char* out = act + (cur ^ 1) * BUFB + b * CFGB;
+    const bool is_conv2 = (l % 2) == 1;
+    const int blk = l / 2;
+    const bool last = (l == 2 * nb - 1);
+    const float scale = last ? rsqrtf((float)(nb + 1)) : (is_conv2 ? rsqrtf((float)(blk + 2)) : 1.0f);
+    float m_w = -INFINITY, s_w = 0.0f;
+
+    Frag Fm = load_row(in, y0 - 1);
+    Frag F0 = load_row(in, y0);
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+      const int y = y0 + i;
+      Frag Fp = load_row(in, y + 1);
+      v4f acc = bias4;
+      if constexpr (MODE == QTX_CONV_FP32) {
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy) {
+          const Frag& R = (dy == 0) ? Fm : (dy == 1 ? F0 : Fp);
+#pragma unroll
+          for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+            for (int jp = 0; jp < 4; ++jp)
+              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(wf[(3 * dy + dx) * 4 + jp], R.f[dx][jp], acc, 0, 0, 0);
+        }
+      } else {
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy) {
+          const Frag& R = (dy == 0) ? Fm : (dy == 1 ? F0 : Fp);
+#pragma unroll
+          for (int pr = 0; pr < 2; ++pr) {
+            const int ks = 2 * dy + pr;
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[ks], R.h[pr], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[ks], R.l[pr], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[ks], R.h[pr], acc, 0, 0, 0);
+          }
+        }
+      }
+      if (!last) {
+        v4f o;
+        if (is_conv2) {
+          v4f xr = acc + res[i];
+          res[i] = xr;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) o[q] = gelu_f(xr[q] * scale, gelu_exact);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) o[q] = gelu_f(acc[q], gelu_exact);
+        }
+        store4(out + y * ROWB, o);
+      } else {
+        float hv[4], m = -INFINITY;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          hv[q] = (acc[q] + res[i][q]) * scale;
+          m = fmaxf(m, hv[q]);
+        }
+        m = wave_max(m);
+        float sacc = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sacc += __expf(hv[q] - m);
+        sacc = wave_sum(sacc);
+        float mm = fmaxf(m_w, m);
+        s_w = s_w * __expf(m_w - mm) + sacc * __expf(m - mm);
+        m_w = mm;
+      }
+      Fm = F0;
+      F0 = Fp;
+    }
+    if (!last) {
+      __syncthreads();
+      cur ^= 1;
+    } else {
+      if (lane == 0) {
+        red[wv * 2] = m_w;
+        red[wv * 2 + 1] = s_w;
+      }
+      __syncthreads();
+      if (threadIdx.x < B) {
+        int bb = threadIdx.x;
+        float m = -INFINITY;
+        for (int w = 4 * bb; w < 4 * bb + 4; ++w) m = fmaxf(m, red[w * 2]);
+        double ssum = 0.0;
+        for (int w = 4 * bb; w < 4 * bb + 4; ++w)
+          ssum += (double)red[w * 2 + 1] * exp((double)red[w * 2] - (double)m);
+        lp_out[bb] = (double)m + log(ssum) - log((double)C * 256.0);
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // ---------------------------------------------------------------- kernels
 struct RcArgs {
   RcGeom g;
@@ -685,7 +916,10 @@ struct RcArgs {
 
 #define RC_FWD(SP, LP)                                                                                        \
   do {                                                                                                        \
-    if constexpr (ROWS)                                                                                       \
+    if constexpr (ROWS == 2)                                                                                  \
+      rc_forward_win<MODE>(a.g, a.nb, a.gelu_exact, a.packed, a.off_l0w, a.off_l0b, a.bias_off, a.off_frag,    \
+                           a.frag_bytes, SP, act, red, LP);                                                   \
+    else if constexpr (ROWS == 1)                                                                             \
       rc_forward_rows<MODE, B, MAXT, W>(a.g, a.nb, a.gelu_exact, a.packed, a.off_l0w, a.off_l0b, a.bias_off,   \
                                         a.off_frag, a.frag_bytes, SP, act, red, LP);                          \
     else                                                                                                      \
@@ -1102,7 +1336,15 @@ static RcArgs rc_args(const RcLayout& L, const void* packed) {
 // (C = 16: N <= 256 -> 2 configurations per workgroup, N <= 512 -> 1; C = 32: N <= 400 -> 1).
 #define RC_DISPATCH(KERNEL, NS_PER_BLOCK_OUT, GRID_FROM_B, ...)                                        \
   do {                                                                                                 \
-    if (L.C == 16 && L.L1 == 16 && L.L0 <= 16) {                                                       \
+    if (L.win) {                                                                                       \
+      constexpr int B_ = 2;                                                                            \
+      NS_PER_BLOCK_OUT = B_;                                                                           \
+      dim3 grid_ = GRID_FROM_B;                                                                        \
+      if (L.mode == QTX_CONV_FP32)                                                                     \
+        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_FP32, 2, 256, 4, 8, 2>), grid_, dim3(512), 0, st, __VA_ARGS__); \
+      else                                                                                             \
+        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_BF16X3, 2, 256, 4, 8, 2>), grid_, dim3(512), 0, st, __VA_ARGS__); \
+    } else if (L.C == 16 && L.L1 == 16 && L.L0 <= 16) {                                                \
       constexpr int B_ = 2;                                                                            \
       NS_PER_BLOCK_OUT = B_;                                                                           \
       dim3 grid_ = GRID_FROM_B;                                                                        \
@@ -1159,21 +1401,24 @@ int qtx_resconv_pack(void* stream, const qtx_resconv_desc* d, const float* theta
   hipLaunchKernelGGL(rc_pack_misc_kernel, dim3((unsigned)((misc + 255) / 256)), dim3(256), 0, st, theta, L.nb, L.C,
                      L.nparams, (float*)packed, L.off_l0w, L.off_l0b, L.bias_off);
   int ct = L.C / 16;
-  int ks = (L.mode == QTX_CONV_FP32) ? 9 * L.C / 4 : (L.C == 16 ? 5 : 9);
+  int ks = (L.mode == QTX_CONV_FP32) ? 9 * L.C / 4 : (L.C == 16 ? (L.win ? 6 : 5) : 9);
   int64_t total = (int64_t)ct * ks * WAVE * (2 * L.nb - 1);
   if (total > 0) {
     dim3 g((unsigned)((total + 255) / 256)), b(256);
     if (L.C == 16 && L.mode == QTX_CONV_FP32)
-      hipLaunchKernelGGL((rc_pack_frag_kernel<16, QTX_CONV_FP32>), g, b, 0, st, theta, L.nb, (char*)packed,
+      hipLaunchKernelGGL((rc_pack_frag_kernel<16, QTX_CONV_FP32, 0>), g, b, 0, st, theta, L.nb, (char*)packed,
+                         L.off_frag, L.frag_bytes);
+    else if (L.C == 16 && L.win)
+      hipLaunchKernelGGL((rc_pack_frag_kernel<16, QTX_CONV_BF16X3, 1>), g, b, 0, st, theta, L.nb, (char*)packed,
                          L.off_frag, L.frag_bytes);
     else if (L.C == 16)
-      hipLaunchKernelGGL((rc_pack_frag_kernel<16, QTX_CONV_BF16X3>), g, b, 0, st, theta, L.nb, (char*)packed,
+      hipLaunchKernelGGL((rc_pack_frag_kernel<16, QTX_CONV_BF16X3, 0>), g, b, 0, st, theta, L.nb, (char*)packed,
                          L.off_frag, L.frag_bytes);
     else if (L.mode == QTX_CONV_FP32)
-      hipLaunchKernelGGL((rc_pack_frag_kernel<32, QTX_CONV_FP32>), g, b, 0, st, theta, L.nb, (char*)packed,
+      hipLaunchKernelGGL((rc_pack_frag_kernel<32, QTX_CONV_FP32, 0>), g, b, 0, st, theta, L.nb, (char*)packed,
                          L.off_frag, L.frag_bytes);
     else
-      hipLaunchKernelGGL((rc_pack_frag_kernel<32, QTX_CONV_BF16X3>), g, b, 0, st, theta, L.nb, (char*)packed,
+      hipLaunchKernelGGL((rc_pack_frag_kernel<32, QTX_CONV_BF16X3, 0>), g, b, 0, st, theta, L.nb, (char*)packed,
                          L.off_frag, L.frag_bytes);
   }
   QTX_LAUNCH_CHECK();

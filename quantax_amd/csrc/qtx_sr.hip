@@ -313,13 +313,39 @@ __global__ __launch_bounds__(1024) void shift_diag_kernel(double* __restrict__ T
   if (tid == 0) scalars[0] = tr;
 }
 
-// soft inverse of eigenvalues times rho (solver.py:94-101), single block
-__global__ __launch_bounds__(1024) void eig_inv_kernel(const double* __restrict__ vals, int64_t n, double rtol,
-                                                       double atol, double* __restrict__ rho) {
+// single block: scale[0] = 2^k with 2^k * max_i |T_ii| in [0.5, 1) (exact scaling)
+__global__ __launch_bounds__(1024) void pow2_scale_kernel(const double* __restrict__ T, int64_t n,
+                                                          double* __restrict__ scale) {
   __shared__ double red[16];
   int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE, nw = blockDim.x / WAVE;
   double m = 0.0;
-  for (int64_t i = tid; i < n; i += blockDim.x) m = fmax(m, fabs(vals[i]));
+  for (int64_t i = tid; i < n; i += blockDim.x) m = fmax(m, fabs(T[i * n + i]));
+  m = wave_max(m);
+  if (lane == 0) red[wv] = m;
+  __syncthreads();
+  if (tid == 0) {
+    double mx = 0.0;
+    for (int w = 0; w < nw; ++w) mx = fmax(mx, red[w]);
+    int e = 0;
+    if (mx > 0.0 && isfinite(mx)) frexp(mx, &e);  // mx = f * 2^e, f in [0.5, 1)
+    scale[0] = ldexp(1.0, -e);
+  }
+}
+
+__global__ void scale_by_kernel(double* __restrict__ T, int64_t n2, const double* __restrict__ scale) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n2) T[i] *= scale[0];
+}
+
+// soft inverse of eigenvalues times rho (solver.py:94-101), single block
+__global__ __launch_bounds__(1024) void eig_inv_kernel(const double* __restrict__ vals, int64_t n, double rtol,
+                                                       double atol, const double* __restrict__ scale,
+                                                       double* __restrict__ rho) {
+  const double inv_scale = 1.0 / scale[0];  // eigenvalues of the scaled matrix -> eigenvalues of T
+  __shared__ double red[16];
+  int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE, nw = blockDim.x / WAVE;
+  double m = 0.0;
+  for (int64_t i = tid; i < n; i += blockDim.x) m = fmax(m, fabs(vals[i] * inv_scale));
   m = wave_max(m);
   if (lane == 0) red[wv] = m;
   __syncthreads();
@@ -327,7 +353,7 @@ __global__ __launch_bounds__(1024) void eig_inv_kernel(const double* __restrict_
   for (int w = 0; w < nw; ++w) vmax = fmax(vmax, red[w]);
   double cut = rtol * vmax + atol;
   for (int64_t i = tid; i < n; i += blockDim.x) {
-    double v = vals[i], va = fabs(v);
+    double v = vals[i] * inv_scale, va = fabs(v);
     double inv = 0.0;
     if (va > 0.0) {
       double q = cut / va;
@@ -483,6 +509,12 @@ static int minsr_solve_t(hipStream_t st, const TI* O, const double* mean, const 
     rc = rb(rocsolver_dpotrs(h, rocblas_fill_lower, ni, 1, w.T, ni, w.y, ni), "rocsolver_dpotrs");
     if (rc) return rc;
   } else {
+    // exact power-of-two rescaling of T (eigenvectors unchanged, eigenvalues
+    // scaled by 2^k and undone in eig_inv_kernel)
+    hipLaunchKernelGGL(pow2_scale_kernel, dim3(1), dim3(1024), 0, st, w.T, n, w.scalars + 1);
+    hipLaunchKernelGGL(scale_by_kernel, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, st, w.T, n * n,
+                       w.scalars + 1);
+    QTX_LAUNCH_CHECK();
     rc = rb(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, ni, w.T, ni, w.D, w.E, w.info),
             "rocsolver_dsyevd");
     if (rc) return rc;
@@ -494,7 +526,7 @@ static int minsr_solve_t(hipStream_t st, const TI* O, const double* mean, const 
     rc = rb(rocblas_dgemv(h, rocblas_operation_transpose, ni, ni, &one, w.T, ni, w.y, 1, &zero, w.z, 1),
             "rocblas_dgemv");
     if (rc) return rc;
-    hipLaunchKernelGGL(eig_inv_kernel, dim3(1), dim3(1024), 0, st, w.D, n, rtol, atol, w.z);
+    hipLaunchKernelGGL(eig_inv_kernel, dim3(1), dim3(1024), 0, st, w.D, n, rtol, atol, w.scalars + 1, w.z);
     QTX_LAUNCH_CHECK();
     rc = rb(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &one, w.T, ni, w.z, 1, &zero, w.y, 1),
             "rocblas_dgemv");
