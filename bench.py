@@ -122,11 +122,14 @@ def run_mode(cfg, mode, args, torch, qtx):
     stages = Stage(torch)
     lr = 1e-2
 
+    last = {}
+
     def vmc_step():
         samples = stages.run("sweep", sampler.sweep)
         ebar = stages.run("eloc", lambda: opt.get_Ebar(samples))
         O, mean = stages.run("jacobian", lambda: opt.get_Obar(samples))
         step = stages.run("solve", lambda: qtx.optimizer.solve_step(O, mean, ebar, solver, None, opt._ws))
+        last.update(samples=samples, O=O, mean=mean, ebar=ebar, step=step, energy=opt.energy)
         stages.run("update", lambda: state.update(lr * step))
         return opt
 
@@ -143,9 +146,45 @@ def run_mode(cfg, mode, args, torch, qtx):
         print(f"[bench] {mode}: timed step {i + 1}/{args.steps}, E = {opt.energy:.6f}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    return dict(ms_per_step=1e3 * dt / args.steps, steps_per_s=args.steps / dt, stages=stages.ms(),
-                energies=energies, acceptance=sampler.acceptance, nconn=float(H._connectivity.float().mean().item()),
-                nparams=state.nparams, sampler=sampler, state=state, H=H)
+    out = dict(ms_per_step=1e3 * dt / args.steps, steps_per_s=args.steps / dt, stages=stages.ms(),
+               energies=energies, acceptance=sampler.acceptance, nconn=float(H._connectivity.float().mean().item()),
+               nparams=state.nparams)
+    if mode == "mixed" and cfg["model"] == "resconv":
+        out["precision"] = precision_report(cfg, state, H, solver, last, torch, qtx)
+    return out
+
+
+def precision_report(cfg, state, H, solver, last, torch, qtx):
+    """Errors of the mixed path against the FP32-network / FP64-Gram path,
+    both evaluated on the last timed step's spins at the current parameters."""
+    model = state.model
+    s = last["samples"].spins
+    la_m = model.logpsi(s)
+    e_m, _ = model.eloc(H, s, la_m)
+    model.set_precision("fp32")
+    la_p = model.logpsi(s)
+    e_p, _ = model.eloc(H, s, la_p)
+    model.set_precision("bf16x3")
+    O = model.jacobian(s)
+    mean = torch.empty(O.shape[1], dtype=torch.float64, device=O.device)
+    from quantax_amd import _lib
+    _lib.call("qtx_colmean", _lib.stream(), _lib.QTX_F32, O.data_ptr(), O.shape[0], O.shape[1], mean.data_ptr())
+
+    def ebar_of(e):
+        eb = torch.empty_like(e)
+        o2 = torch.empty(2, dtype=torch.float64, device=e.device)
+        _lib.call("qtx_ebar", _lib.stream(), e.data_ptr(), None, e.shape[0], eb.data_ptr(), o2.data_ptr())
+        return eb, o2[0].item()
+
+    eb_m, E_m = ebar_of(e_m)
+    eb_p, E_p = ebar_of(e_p)
+    ref_solver = qtx.optimizer.Solver(solver.mode, solver.rshift, solver.ashift, solver.rtol, solver.atol, "fp64")
+    x_m = qtx.optimizer.solve_step(O, mean, eb_m, solver)
+    x_p = qtx.optimizer.solve_step(O, mean, eb_p, ref_solver)
+    rel = (torch.linalg.norm(x_m - x_p) / torch.linalg.norm(x_p)).item()
+    return {"reference": "same spins and parameters: FP32-MFMA network E_loc, FP64-MFMA Gram, FP64 eigh",
+            "update_vector_rel_err": rel, "energy_rel_err": abs(E_m - E_p) / abs(E_p),
+            "logpsi_max_abs_diff": (la_m - la_p).abs().max().item()}
 
 
 def cpu_baseline(cfg, seconds):
@@ -245,6 +284,8 @@ def main():
         out["plain"] = {"value": round(p["steps_per_s"], 5), "ms_per_step": round(p["ms_per_step"], 2),
                         "stages_ms": {k: round(v, 2) for k, v in p["stages"].items()}}
         out["mixed_over_plain"] = round(r["steps_per_s"] / p["steps_per_s"], 3)
+    if "precision" in r:
+        out["precision"] = r["precision"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
     print(json.dumps(out))

@@ -244,10 +244,12 @@ __global__ __launch_bounds__(256) void gram_nt_kernel(const TI* __restrict__ O, 
         const int rq = (G == QTX_GRAM_FP64) ? (lane / 16 + 4 * q) : (4 * (lane / 16) + q);
         int64_t gi = i0 + wr + a * 16 + rq;
         int64_t gj = j0 + wc + b * 16 + lane % 16;
-        if (gi < n && gj < n) {
+        // off-diagonal tiles are mirrored; inside diagonal tiles the lower
+        // triangle is written to both halves so T is exactly symmetric
+        if (gi < n && gj < n && (bi != bj || gi >= gj)) {
           double v = (double)acc[a][b][q];
           T[gi * n + gj] = v;
-          if (bi != bj) T[gj * n + gi] = v;
+          T[gj * n + gi] = v;
         }
       }
 }
