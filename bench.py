@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--sweep-steps", type=int, default=None, help="proposals per chain per sweep (default 2N)")
     p.add_argument("--thermal-sweeps", type=int, default=1)
     p.add_argument("--solver", default="eigh", choices=["eigh", "chol"])
+    p.add_argument("--rtol", type=float, default=1e-4, help="soft pseudo-inverse cutoff of the eigh solver")
+    p.add_argument("--lr", type=float, default=1e-3)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--calib-copy-mb", type=int, default=0,
@@ -110,7 +112,9 @@ def build_problem(cfg, mode, args, qtx):
     sampler = qtx.sampler.SpinExchange(state, cfg["ns"], thermal_steps=args.thermal_sweeps * 2 * N,
                                        sweep_steps=args.sweep_steps, seed=1)
     if args.solver == "eigh":
-        solver = qtx.optimizer.auto_pinv_eig(gram=gram)
+        # soft eigenvalue cutoff at 1e-4 of the largest eigenvalue: the default
+        # 1e-12 gives steps of norm ~1e3 on a freshly initialised deep ResConv
+        solver = qtx.optimizer.auto_pinv_eig(rtol=args.rtol, gram=gram)
     else:
         solver = qtx.optimizer.auto_shift_eig(rshift=0.0, ashift=1e-4, gram=gram)
     return lat, H, state, sampler, solver
@@ -120,7 +124,7 @@ def run_mode(cfg, mode, args, torch, qtx):
     lat, H, state, sampler, solver = build_problem(cfg, mode, args, qtx)
     opt = qtx.optimizer.SR(state, H, solver=solver)
     stages = Stage(torch)
-    lr = 1e-2
+    lr = args.lr
 
     last = {}
 
@@ -129,13 +133,15 @@ def run_mode(cfg, mode, args, torch, qtx):
         ebar = stages.run("eloc", lambda: opt.get_Ebar(samples))
         O, mean = stages.run("jacobian", lambda: opt.get_Obar(samples))
         step = stages.run("solve", lambda: qtx.optimizer.solve_step(O, mean, ebar, solver, None, opt._ws))
-        last.update(samples=samples, O=O, mean=mean, ebar=ebar, step=step, energy=opt.energy)
+        last.update(samples=samples, O=O, mean=mean, ebar=ebar, step=step, energy=opt.energy,
+                    step_norm=float(torch.linalg.norm(step).item()))
         stages.run("update", lambda: state.update(lr * step))
         return opt
 
     for _ in range(args.warmup):
         vmc_step()
-        print(f"[bench] {mode}: warmup step done, E = {opt.energy:.6f}", file=sys.stderr, flush=True)
+        print(f"[bench] {mode}: warmup step done, E = {opt.energy:.6f}, |step| = {last['step_norm']:.3e}",
+              file=sys.stderr, flush=True)
     stages.t.clear()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -143,7 +149,8 @@ def run_mode(cfg, mode, args, torch, qtx):
     for i in range(args.steps):
         vmc_step()
         energies.append(opt.energy)
-        print(f"[bench] {mode}: timed step {i + 1}/{args.steps}, E = {opt.energy:.6f}", file=sys.stderr, flush=True)
+        print(f"[bench] {mode}: timed step {i + 1}/{args.steps}, E = {opt.energy:.6f}, "
+              f"|step| = {last['step_norm']:.3e}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     out = dict(ms_per_step=1e3 * dt / args.steps, steps_per_s=args.steps / dt, stages=stages.ms(),
@@ -272,7 +279,8 @@ def main():
         "dtype": "bf16x3+fp32+fp64" if main_mode == "mixed" else "fp32+fp64",
         "data": "synthetic (He-normal ResConv weights, Philox-seeded chains)",
         "config": {"workload": cfg["workload"], "mode": main_mode, "nsamples": cfg["ns"],
-                   "sweep_steps": sweep_steps, "solver": args.solver, "nparams": r["nparams"]},
+                   "sweep_steps": sweep_steps, "solver": args.solver, "rtol": args.rtol, "lr": args.lr,
+                   "nparams": r["nparams"]},
         "stages_ms": {k: round(v, 2) for k, v in r["stages"].items()},
         "energy": r["energies"][-1],
         "acceptance": round(r["acceptance"], 4),
