@@ -40,6 +40,8 @@ def parse():
     p.add_argument("--solver", default="eigh", choices=["eigh", "chol"])
     p.add_argument("--rtol", type=float, default=1e-4, help="soft pseudo-inverse cutoff of the eigh solver")
     p.add_argument("--lr", type=float, default=1e-3)
+    p.add_argument("--precision-breakdown", action="store_true",
+                   help="also attribute the mixed update-vector error to E_loc and to the Gram")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--calib-copy-mb", type=int, default=0,
@@ -143,6 +145,7 @@ def run_mode(cfg, mode, args, torch, qtx):
         print(f"[bench] {mode}: warmup step done, E = {opt.energy:.6f}, |step| = {last['step_norm']:.3e}",
               file=sys.stderr, flush=True)
     stages.t.clear()
+    sampler.kernel_events.clear()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     energies = []
@@ -153,15 +156,17 @@ def run_mode(cfg, mode, args, torch, qtx):
               f"|step| = {last['step_norm']:.3e}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    kms = [a.elapsed_time(b) for a, b in sampler.kernel_events]
     out = dict(ms_per_step=1e3 * dt / args.steps, steps_per_s=args.steps / dt, stages=stages.ms(),
+               metropolis_kernel_ms=float(np.mean(kms)) if kms else None,
                energies=energies, acceptance=sampler.acceptance, nconn=float(H._connectivity.float().mean().item()),
                nparams=state.nparams)
     if mode == "mixed" and cfg["model"] == "resconv":
-        out["precision"] = precision_report(cfg, state, H, solver, last, torch, qtx)
+        out["precision"] = precision_report(cfg, state, H, solver, last, torch, qtx, args.precision_breakdown)
     return out
 
 
-def precision_report(cfg, state, H, solver, last, torch, qtx):
+def precision_report(cfg, state, H, solver, last, torch, qtx, breakdown=False):
     """Errors of the mixed path against the FP32-network / FP64-Gram path,
     both evaluated on the last timed step's spins at the current parameters."""
     model = state.model
@@ -188,10 +193,16 @@ def precision_report(cfg, state, H, solver, last, torch, qtx):
     ref_solver = qtx.optimizer.Solver(solver.mode, solver.rshift, solver.ashift, solver.rtol, solver.atol, "fp64")
     x_m = qtx.optimizer.solve_step(O, mean, eb_m, solver)
     x_p = qtx.optimizer.solve_step(O, mean, eb_p, ref_solver)
-    rel = (torch.linalg.norm(x_m - x_p) / torch.linalg.norm(x_p)).item()
-    return {"reference": "same spins and parameters: FP32-MFMA network E_loc, FP64-MFMA Gram, FP64 eigh",
-            "update_vector_rel_err": rel, "energy_rel_err": abs(E_m - E_p) / abs(E_p),
-            "logpsi_max_abs_diff": (la_m - la_p).abs().max().item()}
+    def rel(a):
+        return (torch.linalg.norm(a - x_p) / torch.linalg.norm(x_p)).item()
+
+    rep = {"reference": "same spins and parameters: FP32-MFMA network E_loc, FP64-MFMA Gram, FP64 eigh",
+           "update_vector_rel_err": rel(x_m), "energy_rel_err": abs(E_m - E_p) / abs(E_p),
+           "logpsi_max_abs_diff": (la_m - la_p).abs().max().item()}
+    if breakdown:
+        rep["err_from_eloc_only"] = rel(qtx.optimizer.solve_step(O, mean, eb_m, ref_solver))
+        rep["err_from_gram_only"] = rel(qtx.optimizer.solve_step(O, mean, eb_p, solver))
+    return rep
 
 
 def cpu_baseline(cfg, seconds):
@@ -258,12 +269,13 @@ def main():
     if cfg["model"] == "resconv":
         fwd = conv_flops_per_forward(cfg["nblocks"], cfg["channels"], N)
         flops = cfg["ns"] * sweep_steps * fwd  # one Metropolis launch = one sweep
-        t = r["stages"]["sweep"] / 1e3
+        t = r["metropolis_kernel_ms"] / 1e3  # HIP events around the kernel launch, same stream
         ach = flops / t / 1e12
         peak = MFMA_PEAK["bf16"] if main_mode == "mixed" else MFMA_PEAK["fp32"]
         roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(ach / peak, 4), "traffic": None,
-                "kernel": "rc_metropolis_kernel", "flops_per_launch": flops}
+                "kernel": "rc_metropolis_kernel", "flops_per_launch": flops,
+                "avg_launch_ms": round(r["metropolis_kernel_ms"], 3)}
     out = {
         "metric": "VMC steps/s, 16x16 J1J2 ResConv MinSR (mixed vs FP64)" if args.config == "c3" else
         f"VMC steps/s ({args.config})",

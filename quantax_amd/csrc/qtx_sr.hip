@@ -156,6 +156,18 @@ __global__ __launch_bounds__(256) void gram_nt_kernel(const TI* __restrict__ O, 
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = ACC{0, 0, 0, 0};
+  // FP32-accumulating modes flush their MFMA accumulators into FP64 every
+  // FLUSH chunks, so the rounding error does not grow with np
+  constexpr int FLUSH = 8;
+  constexpr bool SHADOW = (G != QTX_GRAM_FP64);
+  v4d dacc[SHADOW ? 4 : 1][SHADOW ? 4 : 1];
+  if constexpr (SHADOW) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) dacc[a][b] = v4d{0, 0, 0, 0};
+  }
+  int chunk = 0;
 
   for (int64_t k0 = 0; k0 < np; k0 += KC) {
     // stage A rows [i0, i0+128) and B rows [j0, j0+128), columns [k0, k0+KC)
@@ -232,6 +244,19 @@ __global__ __launch_bounds__(256) void gram_nt_kernel(const TI* __restrict__ O, 
       }
     }
     __syncthreads();
+    if constexpr (SHADOW) {
+      if (++chunk == FLUSH || k0 + KC >= np) {
+        chunk = 0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) dacc[a][b][q] += (double)acc[a][b][q];
+            acc[a][b] = ACC{0, 0, 0, 0};
+          }
+      }
+    }
   }
   // store, column lane%16 of each 16x16 block; rows of element q:
   //   f32 / bf16 MFMA: 4 * (lane / 16) + q ;  f64 MFMA: lane / 16 + 4 * q
@@ -247,7 +272,11 @@ __global__ __launch_bounds__(256) void gram_nt_kernel(const TI* __restrict__ O, 
         // off-diagonal tiles are mirrored; inside diagonal tiles the lower
         // triangle is written to both halves so T is exactly symmetric
         if (gi < n && gj < n && (bi != bj || gi >= gj)) {
-          double v = (double)acc[a][b][q];
+          double v;
+          if constexpr (SHADOW)
+            v = dacc[a][b][q];
+          else
+            v = (double)acc[a][b][q];
           T[gi * n + gj] = v;
           T[gj * n + gi] = v;
         }

@@ -57,6 +57,7 @@ class Metropolis:
         self.n_near = torch.zeros(self._nsamples, dtype=torch.int32, device=dev)
         self.n_proposed = 0
         self.last_drift = 0
+        self.kernel_events = []  # (start, end) HIP events around each Metropolis kernel launch
         if initial_spins is not None:
             s = torch.as_tensor(initial_spins).to(torch.int8)
             if s.ndim == 1:
@@ -108,8 +109,12 @@ class Metropolis:
         if nsteps > 0:
             _lib.call("qtx_philox_fill", _lib.stream(), self._seed, _lib.TAG_SWEEP, self._step, nsteps, 0, ns,
                       words.data_ptr())
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev0.record()
             model.metropolis(self.kind, self._nbr, self._width, self._hopping, words, nsteps, self._spins,
                              self._logabs, self.n_accept, self.n_near, self._reweight, self._tie_tol)
+            ev1.record()
+            self.kernel_events.append((ev0, ev1))
         self._step += nsteps
         self.n_proposed += nsteps
         uses_ref = self._state.use_ref and hasattr(model, "M")
