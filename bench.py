@@ -37,7 +37,9 @@ def parse():
     p.add_argument("--config", default="c3", choices=["c3", "c2", "c1"])
     p.add_argument("--sweep-steps", type=int, default=None, help="proposals per chain per sweep (default 2N)")
     p.add_argument("--thermal-sweeps", type=int, default=1)
-    p.add_argument("--solver", default="eigh", choices=["eigh", "chol"])
+    p.add_argument("--solver", default="eigh", choices=["eigh", "eigh-direct", "chol"],
+                   help="eigh: soft pseudo-inverse (partial fractions when the cutoff allows); "
+                        "eigh-direct: the same through dsyevd; chol: diagonal-shift Cholesky")
     p.add_argument("--rtol", type=float, default=1e-4, help="soft pseudo-inverse cutoff of the eigh solver")
     p.add_argument("--lr", type=float, default=1e-3)
     p.add_argument("--precision-breakdown", action="store_true",
@@ -113,10 +115,12 @@ def build_problem(cfg, mode, args, qtx):
     state = qtx.state.Variational(model)
     sampler = qtx.sampler.SpinExchange(state, cfg["ns"], thermal_steps=args.thermal_sweeps * 2 * N,
                                        sweep_steps=args.sweep_steps, seed=1)
-    if args.solver == "eigh":
+    if args.solver in ("eigh", "eigh-direct"):
         # soft eigenvalue cutoff at 1e-4 of the largest eigenvalue: the default
         # 1e-12 gives steps of norm ~1e3 on a freshly initialised deep ResConv
         solver = qtx.optimizer.auto_pinv_eig(rtol=args.rtol, gram=gram)
+        if args.solver == "eigh-direct":
+            solver.mode = qtx._lib.QTX_SOLVE_EIGH_PINV_DIRECT
     else:
         solver = qtx.optimizer.auto_shift_eig(rshift=0.0, ashift=1e-4, gram=gram)
     return lat, H, state, sampler, solver
@@ -135,6 +139,7 @@ def run_mode(cfg, mode, args, torch, qtx):
         ebar = stages.run("eloc", lambda: opt.get_Ebar(samples))
         O, mean = stages.run("jacobian", lambda: opt.get_Obar(samples))
         step = stages.run("solve", lambda: qtx.optimizer.solve_step(O, mean, ebar, solver, None, opt._ws))
+        last["solve_info"] = qtx.optimizer.last_solve_info()
         last.update(samples=samples, O=O, mean=mean, ebar=ebar, step=step, energy=opt.energy,
                     step_norm=float(torch.linalg.norm(step).item()))
         stages.run("update", lambda: state.update(lr * step))
@@ -160,7 +165,7 @@ def run_mode(cfg, mode, args, torch, qtx):
     out = dict(ms_per_step=1e3 * dt / args.steps, steps_per_s=args.steps / dt, stages=stages.ms(),
                metropolis_kernel_ms=float(np.mean(kms)) if kms else None,
                energies=energies, acceptance=sampler.acceptance, nconn=float(H._connectivity.float().mean().item()),
-               nparams=state.nparams)
+               nparams=state.nparams, solve_info=last.get("solve_info"))
     if mode == "mixed" and cfg["model"] == "resconv":
         out["precision"] = precision_report(cfg, state, H, solver, last, torch, qtx, args.precision_breakdown)
     return out
@@ -190,7 +195,8 @@ def precision_report(cfg, state, H, solver, last, torch, qtx, breakdown=False):
 
     eb_m, E_m = ebar_of(e_m)
     eb_p, E_p = ebar_of(e_p)
-    ref_solver = qtx.optimizer.Solver(solver.mode, solver.rshift, solver.ashift, solver.rtol, solver.atol, "fp64")
+    ref_mode = qtx._lib.QTX_SOLVE_EIGH_PINV_DIRECT if solver.mode == qtx._lib.QTX_SOLVE_EIGH_PINV else solver.mode
+    ref_solver = qtx.optimizer.Solver(ref_mode, solver.rshift, solver.ashift, solver.rtol, solver.atol, "fp64")
     x_m = qtx.optimizer.solve_step(O, mean, eb_m, solver)
     x_p = qtx.optimizer.solve_step(O, mean, eb_p, ref_solver)
     def rel(a):
@@ -294,6 +300,7 @@ def main():
                    "sweep_steps": sweep_steps, "solver": args.solver, "rtol": args.rtol, "lr": args.lr,
                    "nparams": r["nparams"]},
         "stages_ms": {k: round(v, 2) for k, v in r["stages"].items()},
+        "solve_path": r["solve_info"],
         "energy": r["energies"][-1],
         "acceptance": round(r["acceptance"], 4),
         "mean_nconn": round(r["nconn"], 1),

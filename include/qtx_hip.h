@@ -44,8 +44,9 @@ extern "C" {
 #define QTX_GRAM_BF16X3 2 /* split-bf16: hi*hi + hi*lo + lo*hi, FP32 accum. */
 
 /* solver modes */
-#define QTX_SOLVE_CHOL_SHIFT 0 /* (T + shift I) y = b by Cholesky           */
-#define QTX_SOLVE_EIGH_PINV 1  /* soft pseudo-inverse from eigh(T)          */
+#define QTX_SOLVE_CHOL_SHIFT 0        /* (T + shift I) y = b by Cholesky                */
+#define QTX_SOLVE_EIGH_PINV 1         /* soft pseudo-inverse of T, fastest exact route  */
+#define QTX_SOLVE_EIGH_PINV_DIRECT 2  /* the same soft pseudo-inverse, always by dsyevd */
 
 /* ResConv forward precision */
 #define QTX_CONV_FP32 0   /* FP32 products on FP32 MFMA */
@@ -186,8 +187,14 @@ int qtx_ebar(void* stream, const double* eloc, const double* r, int64_t ns, doub
 /* step = solver(Obar, Ebar) with Obar = (O - mean) sqrt(r/ns) formed on the
  * fly.  ns < np: MinSR, T = Obar Obar^T (ns x ns); else SR, S = Obar^T Obar.
  * mode CHOL_SHIFT: shift = rshift * trace + ashift (solver.py:50-91);
- * mode EIGH_PINV : soft inverse 1/(v (1 + ((rtol max|v| + atol)/|v|)^6))
- *                  (solver.py:94-201, tol_snr = 0).
+ * mode EIGH_PINV : soft inverse g(v) = 1/(v (1 + (c/|v|)^6)), c = rtol max|v| + atol
+ *                  (solver.py:94-201, tol_snr = 0).  When c >= 1e-5 lambda_max it is
+ *                  evaluated without eigenvectors: g(v) = v^5/(v^6 + c^6) =
+ *                  (1/3) sum_k Re 1/(v - p_k), p_k = c exp(i pi (2k+1)/6), each complex
+ *                  shifted system solved by Cholesky of (T - Re p)^2 + (Im p)^2 plus
+ *                  iterative refinement against T; lambda_max by Lanczos.  Smaller
+ *                  cutoffs, or a failed factorisation, use rocSOLVER dsyevd.
+ * mode EIGH_PINV_DIRECT: the same g(T) b through dsyevd always.
  * gram: QTX_GRAM_* (the ns >= np branch always uses FP64 products).
  * ws: qtx_minsr_ws_bytes(...) bytes of device memory. */
 size_t qtx_minsr_ws_bytes(int64_t ns, int64_t np, int mode, int gram);
@@ -198,6 +205,11 @@ int qtx_gram(void* stream, int o_dtype, const void* O, const double* mean, const
 int qtx_minsr_solve(void* stream, int o_dtype, const void* O, const double* mean, const double* r,
                     const double* ebar, int64_t ns, int64_t np, int mode, int gram, double rshift,
                     double ashift, double rtol, double atol, double* step, void* ws, size_t ws_bytes);
+/* What the calling thread's last qtx_minsr_solve did: path 0 Cholesky shift,
+ * 1 partial fractions, 2 dsyevd (-1: none or failed); Lanczos steps and the
+ * lambda_max they gave (0 when not run).  Diagnostic only: no counterpart in
+ * the reference, which always runs eigh. */
+int qtx_minsr_last_path(int* path, int* lanczos_steps, double* lambda_max);
 
 /* theta <- theta - lr * step in dtype, skipped entirely when any step entry is
  * non-finite (state/variational.py:558-579).  *applied (device int32) = 1/0. */

@@ -16,7 +16,7 @@ from .build import LIB
 QTX_F32, QTX_F64 = 0, 1
 QTX_LOCALFLIP, QTX_EXCHANGE = 0, 1
 QTX_GRAM_FP64, QTX_GRAM_FP32, QTX_GRAM_BF16X3 = 0, 1, 2
-QTX_SOLVE_CHOL_SHIFT, QTX_SOLVE_EIGH_PINV = 0, 1
+QTX_SOLVE_CHOL_SHIFT, QTX_SOLVE_EIGH_PINV, QTX_SOLVE_EIGH_PINV_DIRECT = 0, 1, 2
 QTX_CONV_FP32, QTX_CONV_BF16X3 = 0, 1
 TAG_SWEEP, TAG_INIT = 0x51A7, 0x1D17
 
@@ -65,6 +65,7 @@ PROTOS = {
     "qtx_minsr_solve": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, i64, i64, C.c_int, C.c_int, f64, f64, f64, f64, vp,
                                   vp, sz]),
     "qtx_gram": (C.c_int, [vp, C.c_int, vp, vp, vp, i64, i64, C.c_int, vp]),
+    "qtx_minsr_last_path": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(f64)]),
     "qtx_apply_update": (C.c_int, [vp, C.c_int, vp, vp, f64, i64, vp]),
 }
 

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "common.hpp"
 
@@ -417,6 +418,133 @@ __global__ __launch_bounds__(256) void apply_update_kernel(T* __restrict__ theta
   theta[i] = theta[i] - d;
 }
 
+// ---------------------------------------------------------------- soft inverse by partial fractions
+// The soft inverse of solver.py:94-101, g(v) = 1 / (v (1 + (c/|v|)^6)), equals
+// v^5 / (v^6 + c^6).  Its six simple poles p_k = c exp(i pi (2k+1)/6) all have
+// residue 1/6, so pairing conjugates
+//   g(v) = (1/3) sum_{k=0..2} Re 1 / (v - p_k),
+// and g(T) b = (1/3) sum_k Re (T - p_k I)^{-1} b needs no eigendecomposition.
+// Each complex system is solved by iterative refinement whose inner solve is
+// (T - conj(p) I) ((T - a)^2 + b^2)^{-1}, p = a + ib, with the real SPD
+// quadratic factored once by Cholesky; the residual uses T itself, so the
+// result has the accuracy of the well-conditioned complex system.
+
+// sum over the block (every thread gets the total; fixed summation order)
+__device__ __forceinline__ double block_sum(double x, double* red) {
+  int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE, nw = blockDim.x / WAVE;
+  x = wave_sum(x);
+  __syncthreads();  // red may still be read by a previous call
+  if (lane == 0) red[wv] = x;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < nw; ++w) t += red[w];
+  return t;
+}
+
+// single block: V[0..n) = hashed start vector / norm, V[n..3n) = 0, beta[0] = 0
+__global__ __launch_bounds__(1024) void lanczos_init_kernel(int64_t n, double* __restrict__ V,
+                                                            double* __restrict__ beta) {
+  __shared__ double red[16];
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    uint64_t z = (uint64_t)i * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;  // splitmix64
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    double v = (double)(z >> 11) * 0x1.0p-52 - 1.0;  // [-1, 1)
+    V[i] = v;
+    V[n + i] = 0.0;
+    V[2 * n + i] = 0.0;
+    acc += v * v;
+  }
+  double inv = 1.0 / sqrt(block_sum(acc, red));
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) V[i] *= inv;
+  if (threadIdx.x == 0) beta[0] = 0.0;
+}
+
+// single block, after w = T v_j: alpha_j = w.v_j, w -= alpha_j v_j + beta_j v_{j-1},
+// one re-orthogonalisation against v_j, beta_{j+1} = |w|, v_{j+1} = w / beta_{j+1}
+__global__ __launch_bounds__(1024) void lanczos_step_kernel(int64_t n, int j, double* __restrict__ w,
+                                                            const double* __restrict__ v,
+                                                            const double* __restrict__ vp, double* __restrict__ vn,
+                                                            double* __restrict__ alpha, double* __restrict__ beta) {
+  __shared__ double red[16];
+  const double bj = beta[j];
+  double a = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) a += w[i] * v[i];
+  a = block_sum(a, red);
+  double a2 = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    double t = w[i] - a * v[i] - bj * vp[i];
+    w[i] = t;
+    a2 += t * v[i];
+  }
+  a2 = block_sum(a2, red);
+  double nn = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    double t = w[i] - a2 * v[i];
+    w[i] = t;
+    nn += t * t;
+  }
+  double b = sqrt(block_sum(nn, red));
+  double inv = b > 0.0 ? 1.0 / b : 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) vn[i] = w[i] * inv;
+  if (threadIdx.x == 0) {
+    alpha[j] = a + a2;
+    beta[j + 1] = b;
+  }
+}
+
+// A = T2 - 2a T + c^2 I on the lower triangle (column-major, i >= j), 0 above
+__global__ __launch_bounds__(256) void form_quad_kernel(const double* __restrict__ T2,
+                                                        const double* __restrict__ T, int64_t n, double two_a,
+                                                        double c2, double* __restrict__ A) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * n) return;
+  int64_t j = e / n, i = e - j * n;
+  A[e] = (i < j) ? 0.0 : T2[e] - two_a * T[e] + (i == j ? c2 : 0.0);
+}
+
+// R = [b, 0], Z = 0   (R, Z: n x 2 column-major)
+__global__ void refine_init_kernel(const double* __restrict__ rhs, int64_t n, double* __restrict__ R,
+                                   double* __restrict__ Z) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  R[i] = rhs[i];
+  R[n + i] = 0.0;
+  Z[i] = 0.0;
+  Z[n + i] = 0.0;
+}
+
+// Z += (T - conj(p)) W with W = A^{-1} R, TW = T W
+__global__ void refine_update_kernel(const double* __restrict__ W, const double* __restrict__ TW, int64_t n,
+                                     double a, double b, double* __restrict__ Z) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double wr = W[i], wi = W[n + i];
+  Z[i] += TW[i] - a * wr - b * wi;
+  Z[n + i] += TW[n + i] - a * wi + b * wr;
+}
+
+// R = b - (T - p) Z with TZ = T Z
+__global__ void refine_residual_kernel(const double* __restrict__ rhs, const double* __restrict__ Z,
+                                       const double* __restrict__ TZ, int64_t n, double a, double b,
+                                       double* __restrict__ R) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double zr = Z[i], zi = Z[n + i];
+  R[i] = rhs[i] - (TZ[i] - a * zr + b * zi);
+  R[n + i] = -(TZ[n + i] - a * zi - b * zr);
+}
+
+// x (+)= Re Z / 3
+__global__ void refine_accum_kernel(const double* __restrict__ Z, int64_t n, int first, double* __restrict__ x) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = Z[i] * (1.0 / 3.0);
+  x[i] = first ? v : x[i] + v;
+}
+
 // ---------------------------------------------------------------- rocBLAS handle
 static std::mutex g_blas_mu;
 static rocblas_handle g_blas[64] = {nullptr};
@@ -446,6 +574,8 @@ using namespace qtx;
 
 static int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
+constexpr int kLanczosMax = 512;  // Lanczos steps for lambda_max before giving up on convergence
+
 struct SolveWs {
   double* T;       // n x n
   double* y;       // max(ns, np)
@@ -453,10 +583,20 @@ struct SolveWs {
   double* D;       // n
   double* E;       // n
   double* scalars; // 8
-  int32_t* info;   // 1 (+pad)
+  int32_t* info;   // 4 (+pad)
+  // partial-fraction soft inverse (EIGH_PINV modes only)
+  double* T2;      // n x n: T^2 (lower)
+  double* A;       // n x n: (T - a)^2 + b^2, then its Cholesky factor
+  double* R;       // n x 2: residual, then A^{-1} residual
+  double* TW;      // n x 2: T times a pair of vectors
+  double* Z;       // n x 2: complex solution
+  double* x;       // n: accumulated soft-inverse solution
+  double* V;       // 3n: Lanczos vectors
+  double* w;       // n: Lanczos work vector
+  double* ab;      // alpha[kLanczosMax], beta[kLanczosMax + 1]
 };
 
-static size_t ws_layout(int64_t ns, int64_t np, SolveWs* w, char* base) {
+static size_t ws_layout(int64_t ns, int64_t np, int mode, SolveWs* w, char* base) {
   int64_t n = ns < np ? ns : np;
   int64_t m = ns > np ? ns : np;
   size_t off = 0;
@@ -465,15 +605,61 @@ static size_t ws_layout(int64_t ns, int64_t np, SolveWs* w, char* base) {
     off += (bytes + 255) & ~(size_t)255;
     return base ? base + o : nullptr;
   };
-  double* T = (double*)take((size_t)n * n * 8);
-  double* y = (double*)take((size_t)m * 8);
-  double* z = (double*)take((size_t)m * 8);
-  double* D = (double*)take((size_t)n * 8);
-  double* E = (double*)take((size_t)n * 8);
-  double* sc = (double*)take(64);
-  int32_t* info = (int32_t*)take(64);
-  if (w) *w = SolveWs{T, y, z, D, E, sc, info};
+  SolveWs s{};
+  s.T = (double*)take((size_t)n * n * 8);
+  s.y = (double*)take((size_t)m * 8);
+  s.z = (double*)take((size_t)m * 8);
+  s.D = (double*)take((size_t)n * 8);
+  s.E = (double*)take((size_t)n * 8);
+  s.scalars = (double*)take(64);
+  s.info = (int32_t*)take(64);
+  if (mode == QTX_SOLVE_EIGH_PINV) {
+    s.T2 = (double*)take((size_t)n * n * 8);
+    s.A = (double*)take((size_t)n * n * 8);
+    s.R = (double*)take((size_t)n * 16);
+    s.TW = (double*)take((size_t)n * 16);
+    s.Z = (double*)take((size_t)n * 16);
+    s.x = (double*)take((size_t)n * 8);
+    s.V = (double*)take((size_t)n * 24);
+    s.w = (double*)take((size_t)n * 8);
+    s.ab = (double*)take((size_t)(2 * kLanczosMax + 1) * 8);
+  }
+  if (w) *w = s;
   return off;
+}
+
+// what the last qtx_minsr_solve on this thread did
+static thread_local int g_last_path = -1;
+static thread_local int g_last_lanczos = 0;
+static thread_local double g_last_lmax = 0.0;
+
+// largest eigenvalue of the m x m symmetric tridiagonal (diag a[0..m), off-diagonal
+// e[1..m) between rows i-1 and i) by Sturm-count bisection
+static double tridiag_max_eig(const double* a, const double* e, int m) {
+  double lo = a[0], hi = a[0];
+  for (int i = 0; i < m; ++i) {
+    double r = (i > 0 ? fabs(e[i]) : 0.0) + (i + 1 < m ? fabs(e[i + 1]) : 0.0);
+    lo = fmin(lo, a[i] - r);
+    hi = fmax(hi, a[i] + r);
+  }
+  auto count_below = [&](double x) {
+    int c = 0;
+    double q = a[0] - x;
+    for (int i = 0;;) {
+      if (q == 0.0) q = -1e-300;
+      if (q < 0.0) ++c;
+      if (++i >= m) break;
+      q = a[i] - x - e[i] * e[i] / q;
+    }
+    return c;
+  };
+  for (int it = 0; it < 256 && hi - lo > 2e-16 * fmax(fabs(lo), fabs(hi)); ++it) {
+    double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (count_below(mid) == m) hi = mid;
+    else lo = mid;
+  }
+  return hi;
 }
 
 template <typename TI>
@@ -508,6 +694,135 @@ static int launch_obar_t_y(hipStream_t st, const TI* O, const double* mean, cons
   return QTX_OK;
 }
 
+// y <- U g(D) U^T y through rocSOLVER dsyevd (T is overwritten); returns 1 or an error
+static int pinv_syevd(hipStream_t st, rocblas_handle h, int64_t n, double rtol, double atol, SolveWs& w) {
+  const rocblas_int ni = (rocblas_int)n;
+  // exact power-of-two rescaling of T (eigenvectors unchanged, eigenvalues
+  // scaled by 2^k and undone in eig_inv_kernel)
+  hipLaunchKernelGGL(pow2_scale_kernel, dim3(1), dim3(1024), 0, st, w.T, n, w.scalars + 1);
+  hipLaunchKernelGGL(scale_by_kernel, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, st, w.T, n * n,
+                     w.scalars + 1);
+  QTX_LAUNCH_CHECK();
+  int rc = rb(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, ni, w.T, ni, w.D, w.E, w.info),
+              "rocsolver_dsyevd");
+  if (rc) return rc;
+  const double one = 1.0, zero = 0.0;
+  // z = U^T y ; z *= eig_inv ; y = U z   (U column-major in T)
+  rc = rb(rocblas_dgemv(h, rocblas_operation_transpose, ni, ni, &one, w.T, ni, w.y, 1, &zero, w.z, 1),
+          "rocblas_dgemv");
+  if (rc) return rc;
+  hipLaunchKernelGGL(eig_inv_kernel, dim3(1), dim3(1024), 0, st, w.D, n, rtol, atol, w.scalars + 1, w.z);
+  QTX_LAUNCH_CHECK();
+  rc = rb(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &one, w.T, ni, w.z, 1, &zero, w.y, 1), "rocblas_dgemv");
+  if (rc) return rc;
+  int32_t info_h = 0;
+  QTX_HIP(hipMemcpyAsync(&info_h, w.info, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  QTX_HIP(hipStreamSynchronize(st));
+  if (info_h != 0) return fail(QTX_ESOLVER, "syevd info = " + std::to_string(info_h));
+  g_last_path = 2;
+  return 1;
+}
+
+// y <- g(T) y by the partial-fraction form of the soft inverse (T and y are kept
+// until success).  Returns 1 when done, 0 when the caller should use dsyevd
+// (cutoff below 1e-5 lambda_max, non-finite or non-positive spectrum, or a
+// failed Cholesky factorisation), or an error.
+static int pinv_partial_fractions(hipStream_t st, rocblas_handle h, int64_t n, double rtol, double atol,
+                                  SolveWs& w) {
+  const rocblas_int ni = (rocblas_int)n;
+  const double one = 1.0, zero = 0.0;
+  const unsigned gb = (unsigned)nblk(n, 256);
+  int rc;
+  if (atol == 0.0 && !(rtol >= 1e-5)) return 0;  // cutoff too small for the refinement to contract
+  // lambda_max = max |eigenvalue| of the PSD T by Lanczos (three-term recurrence,
+  // convergence of the largest Ritz value checked every 32 steps)
+  double* alpha = w.ab;
+  double* beta = w.ab + kLanczosMax;
+  hipLaunchKernelGGL(lanczos_init_kernel, dim3(1), dim3(1024), 0, st, n, w.V, beta);
+  const int mcap = (int)(n < kLanczosMax ? n : kLanczosMax);
+  std::vector<double> ha(kLanczosMax), hb(kLanczosMax + 1);
+  int done = 0, target = mcap < 48 ? mcap : 48, msteps = 0;
+  double lmax = 0.0;
+  for (;;) {
+    for (int j = done; j < target; ++j) {
+      double* v = w.V + (j % 3) * n;
+      double* vp = w.V + ((j + 2) % 3) * n;
+      double* vn = w.V + ((j + 1) % 3) * n;
+      rc = rb(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &one, w.T, ni, v, 1, &zero, w.w, 1), "rocblas_dgemv");
+      if (rc) return rc;
+      hipLaunchKernelGGL(lanczos_step_kernel, dim3(1), dim3(1024), 0, st, n, j, w.w, v, vp, vn, alpha, beta);
+    }
+    QTX_LAUNCH_CHECK();
+    done = target;
+    QTX_HIP(hipMemcpyAsync(ha.data(), alpha, (size_t)done * 8, hipMemcpyDeviceToHost, st));
+    QTX_HIP(hipMemcpyAsync(hb.data(), beta, (size_t)(done + 1) * 8, hipMemcpyDeviceToHost, st));
+    QTX_HIP(hipStreamSynchronize(st));
+    int m = done;
+    bool exact = false;
+    for (int j = 0; j < done; ++j) {
+      if (!std::isfinite(ha[j]) || !std::isfinite(hb[j + 1])) return 0;
+      if (hb[j + 1] == 0.0) {  // invariant Krylov subspace: the Ritz values are exact
+        m = j + 1;
+        exact = true;
+        break;
+      }
+    }
+    lmax = tridiag_max_eig(ha.data(), hb.data(), m);
+    msteps = m;
+    if (exact || done >= mcap) break;
+    double prev = tridiag_max_eig(ha.data(), hb.data(), m - 8);
+    if (fabs(lmax - prev) <= 1e-14 * fabs(lmax)) break;
+    target = done + 32 < mcap ? done + 32 : mcap;
+  }
+  g_last_lanczos = msteps;
+  g_last_lmax = lmax;
+  if (!(lmax > 0.0) || !std::isfinite(lmax)) return 0;
+  const double c = rtol * lmax + atol, rel = c / lmax;
+  if (!(rel >= 1e-5) || !std::isfinite(c)) return 0;
+  // refinement contracts by about 4 u / rel^2 per pass
+  const int iters = rel >= 1e-3 ? 2 : (rel >= 1e-4 ? 3 : 4);
+  rc = rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, ni, ni, &one, w.T, ni, &zero, w.T2, ni),
+          "rocblas_dsyrk");
+  if (rc) return rc;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, st, w.info, (int64_t)4, 0);
+  for (int k = 0; k < 3; ++k) {
+    const double th = 3.14159265358979323846 * (2 * k + 1) / 6.0;
+    const double a = (k == 1) ? 0.0 : c * cos(th), b = c * sin(th);
+    hipLaunchKernelGGL(form_quad_kernel, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, st, w.T2, w.T, n,
+                       2.0 * a, c * c, w.A);
+    QTX_LAUNCH_CHECK();
+    rc = rb(rocsolver_dpotrf(h, rocblas_fill_lower, ni, w.A, ni, w.info + k), "rocsolver_dpotrf");
+    if (rc) return rc;
+    hipLaunchKernelGGL(refine_init_kernel, dim3(gb), dim3(256), 0, st, w.y, n, w.R, w.Z);
+    for (int it = 0; it < iters; ++it) {
+      rc = rb(rocsolver_dpotrs(h, rocblas_fill_lower, ni, 2, w.A, ni, w.R, ni), "rocsolver_dpotrs");
+      if (rc) return rc;
+      rc = rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, ni, 2, ni, &one, w.T, ni, w.R, ni,
+                            &zero, w.TW, ni),
+              "rocblas_dgemm");
+      if (rc) return rc;
+      hipLaunchKernelGGL(refine_update_kernel, dim3(gb), dim3(256), 0, st, w.R, w.TW, n, a, b, w.Z);
+      if (it + 1 < iters) {
+        rc = rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, ni, 2, ni, &one, w.T, ni, w.Z, ni,
+                              &zero, w.TW, ni),
+                "rocblas_dgemm");
+        if (rc) return rc;
+        hipLaunchKernelGGL(refine_residual_kernel, dim3(gb), dim3(256), 0, st, w.y, w.Z, w.TW, n, a, b, w.R);
+      }
+    }
+    hipLaunchKernelGGL(refine_accum_kernel, dim3(gb), dim3(256), 0, st, w.Z, n, k == 0 ? 1 : 0, w.x);
+    QTX_LAUNCH_CHECK();
+  }
+  int32_t info_h[4] = {0, 0, 0, 0};
+  QTX_HIP(hipMemcpyAsync(info_h, w.info, sizeof(info_h), hipMemcpyDeviceToHost, st));
+  QTX_HIP(hipStreamSynchronize(st));
+  if (info_h[0] || info_h[1] || info_h[2]) return 0;
+  hipLaunchKernelGGL(copy_f64_kernel, dim3(gb), dim3(256), 0, st, w.x, w.y, n);
+  QTX_LAUNCH_CHECK();
+  g_last_path = 1;
+  return 1;
+}
+
 template <typename TI>
 static int minsr_solve_t(hipStream_t st, const TI* O, const double* mean, const double* r, const double* ebar,
                          int64_t ns, int64_t np, int mode, int gram, double rshift, double ashift, double rtol,
@@ -531,45 +846,29 @@ static int minsr_solve_t(hipStream_t st, const TI* O, const double* mean, const 
     rc = launch_obar_t_y<TI>(st, O, mean, r, ebar, ns, np, w.y);  // F = Obar^T ebar
     if (rc) return rc;
   }
-  const rocblas_int ni = (rocblas_int)n;
   if (mode == QTX_SOLVE_CHOL_SHIFT) {
+    const rocblas_int ni = (rocblas_int)n;
     hipLaunchKernelGGL(shift_diag_kernel, dim3(1), dim3(1024), 0, st, w.T, n, rshift, ashift, w.scalars);
     QTX_LAUNCH_CHECK();
     rc = rb(rocsolver_dpotrf(h, rocblas_fill_lower, ni, w.T, ni, w.info), "rocsolver_dpotrf");
     if (rc) return rc;
     rc = rb(rocsolver_dpotrs(h, rocblas_fill_lower, ni, 1, w.T, ni, w.y, ni), "rocsolver_dpotrs");
     if (rc) return rc;
+    int32_t info_h = 0;
+    QTX_HIP(hipMemcpyAsync(&info_h, w.info, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    QTX_HIP(hipStreamSynchronize(st));
+    if (info_h != 0) return fail(QTX_ESOLVER, "potrf info = " + std::to_string(info_h));
+    g_last_path = 0;
   } else {
-    // exact power-of-two rescaling of T (eigenvectors unchanged, eigenvalues
-    // scaled by 2^k and undone in eig_inv_kernel)
-    hipLaunchKernelGGL(pow2_scale_kernel, dim3(1), dim3(1024), 0, st, w.T, n, w.scalars + 1);
-    hipLaunchKernelGGL(scale_by_kernel, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, st, w.T, n * n,
-                       w.scalars + 1);
-    QTX_LAUNCH_CHECK();
-    rc = rb(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, ni, w.T, ni, w.D, w.E, w.info),
-            "rocsolver_dsyevd");
-    if (rc) return rc;
-    const double one = 1.0, zero = 0.0;
     rocblas_pointer_mode pm;
     rocblas_get_pointer_mode(h, &pm);
     rocblas_set_pointer_mode(h, rocblas_pointer_mode_host);
-    // z = U^T y ; z *= eig_inv ; y = U z   (U column-major in T)
-    rc = rb(rocblas_dgemv(h, rocblas_operation_transpose, ni, ni, &one, w.T, ni, w.y, 1, &zero, w.z, 1),
-            "rocblas_dgemv");
-    if (rc) return rc;
-    hipLaunchKernelGGL(eig_inv_kernel, dim3(1), dim3(1024), 0, st, w.D, n, rtol, atol, w.scalars + 1, w.z);
-    QTX_LAUNCH_CHECK();
-    rc = rb(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &one, w.T, ni, w.z, 1, &zero, w.y, 1),
-            "rocblas_dgemv");
+    int done = 0;
+    if (mode == QTX_SOLVE_EIGH_PINV) done = pinv_partial_fractions(st, h, n, rtol, atol, w);
+    if (done == 0) done = pinv_syevd(st, h, n, rtol, atol, w);
     rocblas_set_pointer_mode(h, pm);
-    if (rc) return rc;
+    if (done < 0) return done;
   }
-  int32_t info_h = 0;
-  QTX_HIP(hipMemcpyAsync(&info_h, w.info, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  QTX_HIP(hipStreamSynchronize(st));
-  if (info_h != 0)
-    return fail(QTX_ESOLVER, std::string(mode == QTX_SOLVE_CHOL_SHIFT ? "potrf" : "syevd") +
-                                 " info = " + std::to_string(info_h));
   if (minnorm) {
     rc = launch_obar_t_y<TI>(st, O, mean, r, w.y, ns, np, step);
     if (rc) return rc;
@@ -609,10 +908,9 @@ int qtx_ebar(void* stream, const double* eloc, const double* r, int64_t ns, doub
 }
 
 size_t qtx_minsr_ws_bytes(int64_t ns, int64_t np, int mode, int gram) {
-  (void)mode;
   (void)gram;
   if (ns <= 0 || np <= 0) return 0;
-  return ws_layout(ns, np, nullptr, nullptr);
+  return ws_layout(ns, np, mode, nullptr, nullptr);
 }
 
 int qtx_minsr_solve(void* stream, int o_dtype, const void* O, const double* mean, const double* r,
@@ -620,19 +918,31 @@ int qtx_minsr_solve(void* stream, int o_dtype, const void* O, const double* mean
                     double rtol, double atol, double* step, void* ws, size_t ws_bytes) {
   QTX_REQUIRE(o_dtype == QTX_F32 || o_dtype == QTX_F64, "o_dtype must be QTX_F32 or QTX_F64");
   QTX_REQUIRE(ns > 0 && np > 0 && O && mean && ebar && step, "bad arguments");
-  QTX_REQUIRE(mode == QTX_SOLVE_CHOL_SHIFT || mode == QTX_SOLVE_EIGH_PINV, "unknown solver mode");
+  QTX_REQUIRE(mode == QTX_SOLVE_CHOL_SHIFT || mode == QTX_SOLVE_EIGH_PINV || mode == QTX_SOLVE_EIGH_PINV_DIRECT,
+              "unknown solver mode");
   QTX_REQUIRE(gram == QTX_GRAM_FP64 || gram == QTX_GRAM_FP32 || gram == QTX_GRAM_BF16X3, "unknown gram mode");
   size_t need = qtx_minsr_ws_bytes(ns, np, mode, gram);
   if (ws == nullptr || ws_bytes < need)
     return fail(QTX_EWORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
   SolveWs w;
-  ws_layout(ns, np, &w, (char*)ws);
+  ws_layout(ns, np, mode, &w, (char*)ws);
+  g_last_path = -1;
+  g_last_lanczos = 0;
+  g_last_lmax = 0.0;
   hipStream_t st = as_stream(stream);
   if (o_dtype == QTX_F32)
     return minsr_solve_t<float>(st, (const float*)O, mean, r, ebar, ns, np, mode, gram, rshift, ashift, rtol, atol,
                                 step, w);
   return minsr_solve_t<double>(st, (const double*)O, mean, r, ebar, ns, np, mode, gram, rshift, ashift, rtol, atol,
                                step, w);
+}
+
+int qtx_minsr_last_path(int* path, int* lanczos_steps, double* lambda_max) {
+  QTX_REQUIRE(path && lanczos_steps && lambda_max, "bad arguments");
+  *path = g_last_path;
+  *lanczos_steps = g_last_lanczos;
+  *lambda_max = g_last_lmax;
+  return QTX_OK;
 }
 
 int qtx_gram(void* stream, int o_dtype, const void* O, const double* mean, const double* r, int64_t ns,

@@ -11,6 +11,7 @@ device; nothing of size Ns x Np is materialised beyond O itself.
 """
 from __future__ import annotations
 
+import ctypes as C
 from dataclasses import dataclass
 from typing import Optional
 
@@ -91,6 +92,15 @@ def solve_step(O: torch.Tensor, Omean: torch.Tensor, ebar: torch.Tensor, solver:
               ns, np_, solver.mode, _GRAM[solver.gram], solver.rshift, solver.ashift, solver.rtol, solver.atol,
               step.data_ptr(), ws.data_ptr(), ws.numel())
     return step
+
+
+def last_solve_info() -> dict:
+    """What the last solve on this thread did: ``path`` is "chol_shift",
+    "partial_fractions" or "syevd"; Lanczos steps and lambda_max when run."""
+    path, steps, lmax = C.c_int(), C.c_int(), C.c_double()
+    _lib.call("qtx_minsr_last_path", C.byref(path), C.byref(steps), C.byref(lmax))
+    names = {0: "chol_shift", 1: "partial_fractions", 2: "syevd"}
+    return {"path": names.get(path.value, "none"), "lanczos_steps": steps.value, "lambda_max": lmax.value}
 
 
 class QNGD:

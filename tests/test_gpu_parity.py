@@ -302,6 +302,61 @@ def test_minsr_solve(gram, tol, mode, shape, rng):
     assert err < (1e-10 if ns >= np_ else tol), err
 
 
+def _solve_inputs(O, E):
+    ns, np_ = O.shape
+    Od = to_dev(O)
+    mean = torch.empty(np_, dtype=torch.float64, device=DEV)
+    _lib.call("qtx_colmean", _lib.stream(), _lib.QTX_F32, Od.data_ptr(), ns, np_, mean.data_ptr())
+    eb = torch.empty(ns, dtype=torch.float64, device=DEV)
+    out2 = torch.empty(2, dtype=torch.float64, device=DEV)
+    _lib.call("qtx_ebar", _lib.stream(), to_dev(E).data_ptr(), None, ns, eb.data_ptr(), out2.data_ptr())
+    return Od, mean, eb
+
+
+@pytest.mark.parametrize("rtol", [1e-2, 1e-4, 1e-5])
+@pytest.mark.parametrize("shape", [(200, 1000), (512, 3000), (300, 40)])
+def test_pinv_partial_fractions(rtol, shape, rng):
+    """The eigenvector-free soft inverse against the oracle's eigh form, and
+    the Lanczos lambda_max against numpy's eigvalsh."""
+    ns, np_ = shape
+    O, E = _solver_case(ns, np_, np.float32, rng)
+    O[:, : np_ // 2] *= 0.05  # spread the spectrum
+    Od, mean, eb = _solve_inputs(O, E)
+    Obar, _ = osr.get_Obar(O.astype(np.float64))
+    Ebar, _, _ = osr.get_Ebar(E)
+    ref = osr.auto_pinv_eig(Obar, Ebar, rtol=rtol)
+    solver = qtx.optimizer.auto_pinv_eig(rtol=rtol, gram="fp64")
+    step = qtx.optimizer.solve_step(Od, mean, eb, solver).cpu().numpy()
+    info = qtx.optimizer.last_solve_info()
+    assert info["path"] == "partial_fractions", info
+    Tm = Obar @ Obar.T if ns < np_ else Obar.T @ Obar
+    lmax = np.linalg.eigvalsh(Tm).max()
+    assert abs(info["lambda_max"] - lmax) < 1e-12 * lmax, (info, lmax)
+    err = np.linalg.norm(step - ref) / np.linalg.norm(ref)
+    assert err < 1e-10, err
+    # forced dsyevd on the same inputs agrees as well
+    solver.mode = _lib.QTX_SOLVE_EIGH_PINV_DIRECT
+    step2 = qtx.optimizer.solve_step(Od, mean, eb, solver).cpu().numpy()
+    assert qtx.optimizer.last_solve_info()["path"] == "syevd"
+    assert np.linalg.norm(step2 - ref) / np.linalg.norm(ref) < 1e-10
+
+
+def test_pinv_small_cutoff_uses_syevd(rng):
+    O, E = _solver_case(128, 500, np.float32, rng)
+    Od, mean, eb = _solve_inputs(O, E)
+    qtx.optimizer.solve_step(Od, mean, eb, qtx.optimizer.auto_pinv_eig(rtol=1e-8))
+    assert qtx.optimizer.last_solve_info()["path"] == "syevd"
+
+
+def test_pinv_zero_gram(rng):
+    """Identical samples: Obar = 0, every eigenvalue is 0 and the step is 0."""
+    O = np.tile(rng.standard_normal((1, 700)).astype(np.float32), (64, 1))
+    E = rng.standard_normal(64)
+    Od, mean, eb = _solve_inputs(O, E)
+    step = qtx.optimizer.solve_step(Od, mean, eb, qtx.optimizer.auto_pinv_eig(rtol=1e-3)).cpu().numpy()
+    assert np.all(step == 0.0)
+
+
 def test_apply_update_skips_nonfinite():
     qtx.sites.Chain(8)
     model = qtx.model.RBM_Dense(4, dtype=np.float32, init_std=0.1, seed=1)
