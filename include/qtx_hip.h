@@ -199,9 +199,12 @@ int qtx_ebar(void* stream, const double* eloc, const double* r, int64_t ns, doub
  * ws: qtx_minsr_ws_bytes(...) bytes of device memory. */
 size_t qtx_minsr_ws_bytes(int64_t ns, int64_t np, int mode, int gram);
 /* The Gram matrix alone, FP64 row-major: T = Obar Obar^T (ns x ns) when
- * ns < np, else S = Obar^T Obar (np x np) (solver.py:134-139, 157). */
+ * ns < np, else S = Obar^T Obar (np x np) (solver.py:134-139, 157).
+ * ws: qtx_gram_ws_bytes(...) bytes (the bf16 hi/lo planes of the split-bf16
+ * mode; 0 bytes, ws may be NULL, otherwise). */
+size_t qtx_gram_ws_bytes(int64_t ns, int64_t np, int gram);
 int qtx_gram(void* stream, int o_dtype, const void* O, const double* mean, const double* r, int64_t ns,
-             int64_t np, int gram, double* T);
+             int64_t np, int gram, double* T, void* ws, size_t ws_bytes);
 int qtx_minsr_solve(void* stream, int o_dtype, const void* O, const double* mean, const double* r,
                     const double* ebar, int64_t ns, int64_t np, int mode, int gram, double rshift,
                     double ashift, double rtol, double atol, double* step, void* ws, size_t ws_bytes);

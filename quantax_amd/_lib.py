@@ -64,7 +64,8 @@ PROTOS = {
     "qtx_minsr_ws_bytes": (sz, [i64, i64, C.c_int, C.c_int]),
     "qtx_minsr_solve": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, i64, i64, C.c_int, C.c_int, f64, f64, f64, f64, vp,
                                   vp, sz]),
-    "qtx_gram": (C.c_int, [vp, C.c_int, vp, vp, vp, i64, i64, C.c_int, vp]),
+    "qtx_gram_ws_bytes": (sz, [i64, i64, C.c_int]),
+    "qtx_gram": (C.c_int, [vp, C.c_int, vp, vp, vp, i64, i64, C.c_int, vp, vp, sz]),
     "qtx_minsr_last_path": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(f64)]),
     "qtx_apply_update": (C.c_int, [vp, C.c_int, vp, vp, f64, i64, vp]),
 }

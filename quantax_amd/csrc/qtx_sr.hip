@@ -113,11 +113,6 @@ struct GramCfg<QTX_GRAM_FP32> {
   static constexpr int PAD = 1;   // LDS row padding (elements)
 };
 template <>
-struct GramCfg<QTX_GRAM_BF16X3> {
-  static constexpr int KC = 32;
-  static constexpr int PAD = 8;
-};
-template <>
 struct GramCfg<QTX_GRAM_FP64> {
   static constexpr int KC = 16;
   static constexpr int PAD = 1;
@@ -139,11 +134,8 @@ __global__ __launch_bounds__(256) void gram_nt_kernel(const TI* __restrict__ O, 
   constexpr int KC = GramCfg<G>::KC;
   constexpr int LD = KC + GramCfg<G>::PAD;
   using ST = typename std::conditional<G == QTX_GRAM_FP64, double, float>::type;
-  // LDS: FP32/FP64 modes stage ST values; BF16X3 stages hi and lo planes
-  constexpr int PLANES = (G == QTX_GRAM_BF16X3) ? 2 : 1;
-  using LT = typename std::conditional<G == QTX_GRAM_BF16X3, __bf16, ST>::type;
-  __shared__ LT As[PLANES][GT * LD];
-  __shared__ LT Bs[PLANES][GT * LD];
+  __shared__ ST As[1][GT * LD];
+  __shared__ ST Bs[1][GT * LD];
 
   int bi, bj;
   tile_of(blockIdx.x, bi, bj);
@@ -157,8 +149,8 @@ __global__ __launch_bounds__(256) void gram_nt_kernel(const TI* __restrict__ O, 
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = ACC{0, 0, 0, 0};
-  // FP32-accumulating modes flush their MFMA accumulators into FP64 every
-  // FLUSH chunks, so the rounding error does not grow with np
+  // the FP32 mode flushes its MFMA accumulators into FP64 every FLUSH chunks,
+  // so the rounding error does not grow with np
   constexpr int FLUSH = 8;
   constexpr bool SHADOW = (G != QTX_GRAM_FP64);
   v4d dacc[SHADOW ? 4 : 1][SHADOW ? 4 : 1];
@@ -183,49 +175,11 @@ __global__ __launch_bounds__(256) void gram_nt_kernel(const TI* __restrict__ O, 
           double sc = sqrt((r ? r[gr] : 1.0) * inv_ns);
           v = ((double)O[gr * np + k] - mean[k]) * sc;
         }
-        LT* dst = (which == 0) ? As[0] : Bs[0];
-        if constexpr (G == QTX_GRAM_BF16X3) {
-          float f = (float)v;
-          __bf16 hi = (__bf16)f;
-          __bf16 lo = (__bf16)(f - (float)hi);
-          dst[row * LD + kk] = hi;
-          ((which == 0) ? As[1] : Bs[1])[row * LD + kk] = lo;
-        } else {
-          dst[row * LD + kk] = (ST)v;
-        }
+        ((which == 0) ? As[0] : Bs[0])[row * LD + kk] = (ST)v;
       }
     }
     __syncthreads();
-    if constexpr (G == QTX_GRAM_BF16X3) {
-      // one K=32 step: 3 MFMAs per 16x16 block
-      v8bf ah[4], al[4], bh[4], bl[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const int row = wr + a * 16 + lane % 16, kq = 8 * (lane / 16);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          ah[a][q] = As[0][row * LD + kq + q];
-          al[a][q] = As[1][row * LD + kq + q];
-        }
-      }
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const int row = wc + b * 16 + lane % 16, kq = 8 * (lane / 16);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          bh[b][q] = Bs[0][row * LD + kq + q];
-          bl[b][q] = Bs[1][row * LD + kq + q];
-        }
-      }
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
-        }
-    } else {
+    {
 #pragma unroll
       for (int ks = 0; ks < KC; ks += 4) {
         ST av[4], bv[4];
@@ -280,6 +234,157 @@ __global__ __launch_bounds__(256) void gram_nt_kernel(const TI* __restrict__ O, 
             v = (double)acc[a][b][q];
           T[gi * n + gj] = v;
           T[gj * n + gi] = v;
+        }
+      }
+}
+
+// ---------------------------------------------------------------- split-bf16 Gram
+// Pre-pass: Obar = (O - mean) sqrt(r / ns) in FP64, rounded to FP32 x and
+// split into bf16 planes hi = bf16(x), lo = bf16(x - hi), row-major with rows
+// padded to a multiple of GT and K to a multiple of G3_KC (zeros).  The Gram
+// kernel then streams the planes straight into LDS and does only MFMA work:
+// T_ij = sum_k lo_i hi_j + hi_i lo_j + hi_i hi_j (lo * lo dropped).
+constexpr int G3_KC = 64;            // K per LDS stage (two 16x16x32 MFMA steps)
+constexpr int G3_FLUSH = 4;          // stages per FP32 -> FP64 flush (256 K)
+constexpr int G3_PLANE = GT * G3_KC * 2;  // bytes of one bf16 plane of one stage (16 KB)
+
+static inline int64_t g3_rows(int64_t n) { return (n + GT - 1) / GT * GT; }
+static inline int64_t g3_kpad(int64_t np) { return (np + G3_KC - 1) / G3_KC * G3_KC; }
+static inline size_t g3_bytes(int64_t n, int64_t np) { return (size_t)g3_rows(n) * g3_kpad(np) * 2 * sizeof(__bf16); }
+
+template <typename TI>
+__global__ __launch_bounds__(256) void split_bf16x3_kernel(const TI* __restrict__ O, const double* __restrict__ mean,
+                                                           const double* __restrict__ r, int64_t n, int64_t np,
+                                                           int64_t kpad, int64_t bpr, double inv_ns,
+                                                           __bf16* __restrict__ Hi, __bf16* __restrict__ Lo) {
+  const int64_t row = blockIdx.x / bpr;
+  const int64_t k = (blockIdx.x % bpr) * 256 + threadIdx.x;
+  if (k >= kpad) return;
+  double v = 0.0;
+  if (row < n && k < np) v = ((double)O[row * np + k] - mean[k]) * sqrt((r ? r[row] : 1.0) * inv_ns);
+  const float f = (float)v;
+  const __bf16 hi = (__bf16)f;
+  Hi[row * kpad + k] = hi;
+  Lo[row * kpad + k] = (__bf16)(f - (float)hi);
+}
+
+// lower-triangle tile t of the XCD-contiguous order: workgroups are dealt to
+// the 8 XCDs round-robin, so block b runs on XCD b % 8; giving each XCD a
+// contiguous run of tiles (shared row panels) keeps them in its L2
+__device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
+  const int per = ntiles / 8, rem = ntiles % 8, x = bid % 8, loc = bid / 8;
+  return x * per + (x < rem ? x : rem) + loc;
+}
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// 128 x 128 tile, 4 waves (2 x 2, 64 x 64 each).  Operands move HBM -> LDS by
+// LDS DMA (global_load_lds_dwordx4: lane l of a wave writes 16 bytes at the
+// wave's LDS base + 16 l), two 64 KB stages so chunk c + 1 is in flight while
+// chunk c is multiplied.  Wave w fetches plane w (A hi, A lo, B hi, B lo).  A
+// plane is 128 rows x 128 bytes; the 16-byte segment s of row r sits in slot
+// s ^ (r & 7), so the MFMA fragment reads (16 rows, one segment) hit 8
+// distinct slots per 8 lanes: no bank conflicts.  The stage barriers are raw
+// s_barrier with explicit vmcnt waits: a __syncthreads() fence would drain
+// the next chunk's DMA as well.
+__global__ __launch_bounds__(256) void gram_bf16x3_kernel(const __bf16* __restrict__ Hi,
+                                                          const __bf16* __restrict__ Lo, int64_t n, int64_t kpad,
+                                                          int ntiles, double* __restrict__ T) {
+  __shared__ __attribute__((aligned(16))) char smem[2 * 4 * G3_PLANE];
+  int bi, bj;
+  tile_of(xcd_tile(blockIdx.x, ntiles), bi, bj);
+  const int64_t i0 = (int64_t)bi * GT, j0 = (int64_t)bj * GT;
+  const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
+  const int wr = (wv / 2) * 64, wc = (wv % 2) * 64;
+  const __bf16* src = (wv == 0) ? Hi + i0 * kpad : (wv == 1) ? Lo + i0 * kpad : (wv == 2) ? Hi + j0 * kpad : Lo + j0 * kpad;
+  // one DMA instruction covers rows 8q .. 8q+7: lane l -> row 8q + l/8, slot l%8,
+  // which holds logical segment (l%8) ^ (l/8)
+  const int lrow = lane / 8, lseg = (lane % 8) ^ (lane / 8);
+  auto issue = [&](int64_t k0, int stage) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const __bf16* g = src + (int64_t)(q * 8 + lrow) * kpad + k0 + lseg * 8;
+      char* dst = smem + (stage * 4 + wv) * G3_PLANE + q * 1024;
+      __builtin_amdgcn_global_load_lds((const void*)g, (lds_void*)dst, 16, 0, 0);
+    }
+  };
+
+  v4f acc[4][4];
+  v4d dacc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      acc[a][b] = v4f{0, 0, 0, 0};
+      dacc[a][b] = v4d{0, 0, 0, 0};
+    }
+  const int nchunks = (int)(kpad / G3_KC);
+  issue(0, 0);
+  if (nchunks > 1) issue(G3_KC, 1);
+  for (int c = 0; c < nchunks; ++c) {
+    // this wave's DMA for chunk c is done (16 newer ones may be in flight), then
+    // every wave's is
+    if (c + 1 < nchunks)
+      asm volatile("s_waitcnt vmcnt(16)\n\ts_barrier" ::: "memory");
+    else
+      asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
+    const char* stg = smem + (c & 1) * 4 * G3_PLANE;
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      v8bf ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int row = wr + a * 16 + lane % 16, slot = (ks * 4 + lane / 16) ^ (row & 7);
+        ah[a] = *(const v8bf*)(stg + 0 * G3_PLANE + row * 128 + slot * 16);
+        al[a] = *(const v8bf*)(stg + 1 * G3_PLANE + row * 128 + slot * 16);
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int row = wc + b * 16 + lane % 16, slot = (ks * 4 + lane / 16) ^ (row & 7);
+        bh[b] = *(const v8bf*)(stg + 2 * G3_PLANE + row * 128 + slot * 16);
+        bl[b] = *(const v8bf*)(stg + 3 * G3_PLANE + row * 128 + slot * 16);
+      }
+      // the three products of a block are 16 MFMAs apart (no back-to-back dependency)
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+    }
+    if ((c + 1) % G3_FLUSH == 0 || c + 1 == nchunks) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) dacc[a][b][q] += (double)acc[a][b][q];
+          acc[a][b] = v4f{0, 0, 0, 0};
+        }
+    }
+    // every wave has read this stage (its fragments fed the MFMAs above)
+    // before the DMA of chunk c + 2 overwrites it
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    if (c + 2 < nchunks) issue((int64_t)(c + 2) * G3_KC, c & 1);
+  }
+  // element q of a 16x16 block: row 4 * (lane / 16) + q, column lane % 16
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int64_t gi = i0 + wr + a * 16 + 4 * (lane / 16) + q;
+        const int64_t gj = j0 + wc + b * 16 + lane % 16;
+        if (gi < n && gj < n && (bi != bj || gi >= gj)) {
+          T[gi * n + gj] = dacc[a][b][q];
+          T[gj * n + gi] = dacc[a][b][q];
         }
       }
 }
@@ -594,9 +699,10 @@ struct SolveWs {
   double* V;       // 3n: Lanczos vectors
   double* w;       // n: Lanczos work vector
   double* ab;      // alpha[kLanczosMax], beta[kLanczosMax + 1]
+  void* split;     // bf16 hi / lo planes of Obar (split-bf16 Gram only)
 };
 
-static size_t ws_layout(int64_t ns, int64_t np, int mode, SolveWs* w, char* base) {
+static size_t ws_layout(int64_t ns, int64_t np, int mode, int gram, SolveWs* w, char* base) {
   int64_t n = ns < np ? ns : np;
   int64_t m = ns > np ? ns : np;
   size_t off = 0;
@@ -624,6 +730,7 @@ static size_t ws_layout(int64_t ns, int64_t np, int mode, SolveWs* w, char* base
     s.w = (double*)take((size_t)n * 8);
     s.ab = (double*)take((size_t)(2 * kLanczosMax + 1) * 8);
   }
+  if (gram == QTX_GRAM_BF16X3 && ns < np) s.split = take(g3_bytes(ns, np));
   if (w) *w = s;
   return off;
 }
@@ -664,18 +771,27 @@ static double tridiag_max_eig(const double* a, const double* e, int m) {
 
 template <typename TI>
 static int launch_gram_nt(hipStream_t st, int gram, const TI* O, const double* mean, const double* r, int64_t n,
-                          int64_t np, double* T) {
+                          int64_t np, double* T, void* split_ws) {
   int64_t nt = (n + GT - 1) / GT;
   int64_t ntiles = nt * (nt + 1) / 2;
   QTX_REQUIRE(ntiles < 0x7fffffff, "Gram too large");
   dim3 g((unsigned)ntiles), bl(256);
   double inv_ns = 1.0 / (double)n;
-  if (gram == QTX_GRAM_BF16X3)
-    hipLaunchKernelGGL((gram_nt_kernel<TI, QTX_GRAM_BF16X3>), g, bl, 0, st, O, mean, r, n, np, inv_ns, T);
-  else if (gram == QTX_GRAM_FP32)
+  if (gram == QTX_GRAM_BF16X3) {
+    QTX_REQUIRE(split_ws != nullptr, "split-bf16 Gram needs its workspace");
+    const int64_t rows = g3_rows(n), kpad = g3_kpad(np), bpr = (kpad + 255) / 256;
+    QTX_REQUIRE(rows * bpr < 0x7fffffff, "Gram too large");
+    __bf16* Hi = (__bf16*)split_ws;
+    __bf16* Lo = Hi + rows * kpad;
+    hipLaunchKernelGGL(split_bf16x3_kernel<TI>, dim3((unsigned)(rows * bpr)), dim3(256), 0, st, O, mean, r, n, np,
+                       kpad, bpr, inv_ns, Hi, Lo);
+    hipLaunchKernelGGL(gram_bf16x3_kernel, g, bl, 0, st, (const __bf16*)Hi, (const __bf16*)Lo, n, kpad, (int)ntiles,
+                       T);
+  } else if (gram == QTX_GRAM_FP32) {
     hipLaunchKernelGGL((gram_nt_kernel<TI, QTX_GRAM_FP32>), g, bl, 0, st, O, mean, r, n, np, inv_ns, T);
-  else
+  } else {
     hipLaunchKernelGGL((gram_nt_kernel<TI, QTX_GRAM_FP64>), g, bl, 0, st, O, mean, r, n, np, inv_ns, T);
+  }
   QTX_LAUNCH_CHECK();
   return QTX_OK;
 }
@@ -836,7 +952,7 @@ static int minsr_solve_t(hipStream_t st, const TI* O, const double* mean, const 
   QTX_REQUIRE(n <= 0x7fffffff, "matrix too large for rocSOLVER");
   // Gram
   if (minnorm) {
-    rc = launch_gram_nt<TI>(st, gram, O, mean, r, ns, np, w.T);
+    rc = launch_gram_nt<TI>(st, gram, O, mean, r, ns, np, w.T, w.split);
     if (rc) return rc;
     hipLaunchKernelGGL(copy_f64_kernel, dim3(nblk(ns, 256)), dim3(256), 0, st, ebar, w.y, ns);
   } else {
@@ -908,9 +1024,8 @@ int qtx_ebar(void* stream, const double* eloc, const double* r, int64_t ns, doub
 }
 
 size_t qtx_minsr_ws_bytes(int64_t ns, int64_t np, int mode, int gram) {
-  (void)gram;
   if (ns <= 0 || np <= 0) return 0;
-  return ws_layout(ns, np, mode, nullptr, nullptr);
+  return ws_layout(ns, np, mode, gram, nullptr, nullptr);
 }
 
 int qtx_minsr_solve(void* stream, int o_dtype, const void* O, const double* mean, const double* r,
@@ -925,7 +1040,7 @@ int qtx_minsr_solve(void* stream, int o_dtype, const void* O, const double* mean
   if (ws == nullptr || ws_bytes < need)
     return fail(QTX_EWORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
   SolveWs w;
-  ws_layout(ns, np, mode, &w, (char*)ws);
+  ws_layout(ns, np, mode, gram, &w, (char*)ws);
   g_last_path = -1;
   g_last_lanczos = 0;
   g_last_lmax = 0.0;
@@ -945,15 +1060,23 @@ int qtx_minsr_last_path(int* path, int* lanczos_steps, double* lambda_max) {
   return QTX_OK;
 }
 
+size_t qtx_gram_ws_bytes(int64_t ns, int64_t np, int gram) {
+  if (ns <= 0 || np <= 0 || gram != QTX_GRAM_BF16X3 || ns >= np) return 0;
+  return g3_bytes(ns, np);
+}
+
 int qtx_gram(void* stream, int o_dtype, const void* O, const double* mean, const double* r, int64_t ns,
-             int64_t np, int gram, double* T) {
+             int64_t np, int gram, double* T, void* ws, size_t ws_bytes) {
   QTX_REQUIRE(o_dtype == QTX_F32 || o_dtype == QTX_F64, "o_dtype must be QTX_F32 or QTX_F64");
   QTX_REQUIRE(ns > 0 && np > 0 && O && mean && T, "bad arguments");
   QTX_REQUIRE(gram == QTX_GRAM_FP64 || gram == QTX_GRAM_FP32 || gram == QTX_GRAM_BF16X3, "unknown gram mode");
+  const size_t need = qtx_gram_ws_bytes(ns, np, gram);
+  if (need > 0 && (ws == nullptr || ws_bytes < need))
+    return fail(QTX_EWORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
   hipStream_t st = as_stream(stream);
   if (ns < np) {
-    if (o_dtype == QTX_F32) return launch_gram_nt<float>(st, gram, (const float*)O, mean, r, ns, np, T);
-    return launch_gram_nt<double>(st, gram, (const double*)O, mean, r, ns, np, T);
+    if (o_dtype == QTX_F32) return launch_gram_nt<float>(st, gram, (const float*)O, mean, r, ns, np, T, ws);
+    return launch_gram_nt<double>(st, gram, (const double*)O, mean, r, ns, np, T, ws);
   }
   dim3 g(nblk(np, 16), nblk(np, 16));
   if (o_dtype == QTX_F32)

@@ -430,7 +430,7 @@ def test_resconv_rows_metropolis_16x16(rng):
 
 
 @pytest.mark.parametrize("gram,tol", [("fp64", 1e-12), ("fp32", 2e-6), ("bf16x3", 1e-5)])
-@pytest.mark.parametrize("shape", [(300, 1000), (256, 20000)])
+@pytest.mark.parametrize("shape", [(300, 1000), (256, 20000), (1100, 5000)])
 def test_gram_entry_point(gram, tol, shape, rng):
     ns, np_ = shape
     O = (rng.standard_normal((ns, np_)) * 0.2 + 0.05).astype(np.float32)
@@ -439,7 +439,11 @@ def test_gram_entry_point(gram, tol, shape, rng):
     _lib.call("qtx_colmean", _lib.stream(), _lib.QTX_F32, Od.data_ptr(), ns, np_, mean.data_ptr())
     T = torch.empty((ns, ns), dtype=torch.float64, device=DEV)
     g = {"fp64": 0, "fp32": 1, "bf16x3": 2}[gram]
-    _lib.call("qtx_gram", _lib.stream(), _lib.QTX_F32, Od.data_ptr(), mean.data_ptr(), None, ns, np_, g, T.data_ptr())
+    nws = int(_lib.lib().qtx_gram_ws_bytes(ns, np_, g))
+    assert (nws > 0) == (gram == "bf16x3")
+    ws = torch.empty(max(nws, 1), dtype=torch.uint8, device=DEV)
+    _lib.call("qtx_gram", _lib.stream(), _lib.QTX_F32, Od.data_ptr(), mean.data_ptr(), None, ns, np_, g, T.data_ptr(),
+              ws.data_ptr(), nws)
     Obar, _ = osr.get_Obar(O.astype(np.float64))
     ref = Obar @ Obar.T
     Tn = T.cpu().numpy()
