@@ -18,6 +18,7 @@
 
 #include <cmath>
 #include <string>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -694,10 +695,29 @@ __device__ __attribute__((always_inline)) inline void rc_forward_rows(const RcGe
 // input row (split-bf16 K-steps 2 dy + pr: pr 0 = taps (dy, 0) + (dy, 1),
 // pr 1 = tap (dy, 2) + a zero pad; FP32 K-steps = taps), so each row is read
 // from LDS once per wave and layer instead of three times.
+//
+// Issue structure: the GELU form and the conv kind (conv1 / conv2 with
+// residual) are compile-time inside a layer (one branch per layer), and the
+// rows are software-pipelined: row i's MFMA chain and row i - 1's epilogue
+// (residual, GELU, bf16 split, LDS store) are one basic block, so VALU and
+// MFMA work of the same wave can overlap.  Layer l + 1's weight fragments
+// travel HBM/L2 -> LDS by DMA while layer l runs (wst: two stages of
+// RC_WSTAGE bytes) and are read from LDS at the layer start.  The residual
+// stream x (FP32, 16 bytes per lane and row) lives in wst after the stages,
+// in slots private to each lane, which keeps the pipelined loop in registers.
+constexpr int RC_WSTAGE = 12288;  // one layer's fragments: bf16x3 6 x 64 x 32 B; FP32 9216 B
+constexpr int RC_WIN_LDS = 2 * RC_WSTAGE + 2 * 16 * 64 * 16;  // weight stages + residual slots
+typedef __attribute__((address_space(3))) void rc_lds_void;
+
+template <bool GE>
+__device__ __forceinline__ float gelu_t(float x) {
+  return gelu_f(x, GE ? 1 : 0);
+}
+
 template <int MODE>
 __device__ __attribute__((always_inline)) inline void rc_forward_win(
     const RcGeom& g, int nb, int gelu_exact, const char* __restrict__ packed, int64_t off_l0w, int64_t off_l0b,
-    int64_t bias_off, int64_t off_frag, int64_t frag_bytes, const int8_t* sp, char* act, float* red,
+    int64_t bias_off, int64_t off_frag, int64_t frag_bytes, const int8_t* sp, char* act, char* wst, float* red,
     double* lp_out) {
   constexpr int C = 16, B = 2, W = 8, L0 = 16, RPW = 4;
   constexpr int ROWB = 16 * 64;
@@ -764,8 +784,18 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
     }
     return F;
   };
+  // layer l's fragment block -> weight stage l & 1, frag_bytes / 1024 wave-wide DMA instructions
+  auto stage_weights = [&](int l) {
+    const char* src = packed + off_frag + (int64_t)l * frag_bytes;
+    char* dst = wst + (l & 1) * RC_WSTAGE;
+    const int nchunk = (int)(frag_bytes >> 10);
+    for (int c = wv; c < nchunk; c += W)
+      __builtin_amdgcn_global_load_lds((const void*)(src + c * 1024 + lane * 16), (rc_lds_void*)(dst + c * 1024), 16,
+                                       0, 0);
+  };
 
-  v4f res[RPW];
+  v4f* resv = reinterpret_cast<v4f*>(wst + 2 * RC_WSTAGE) + (b * 16 + y0) * WAVE + lane;  // row i: resv[i * WAVE]
+  stage_weights(1);
   // ---- layer 0 on VALU
   {
     const float* W0 = pf + off_l0w;
@@ -789,120 +819,147 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
         for (int tap = 0; tap < 9; ++tap) a += W0[co * 9 + tap] * sv[tap];
         z[q] = gelu_f(a, gelu_exact);
       }
-      res[i] = v4f{spin, spin, spin, spin};
+      resv[i * WAVE] = v4f{spin, spin, spin, spin};
       store4(act + b * CFGB + y * ROWB, z);
     }
   }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wave's weight DMA has landed
   __syncthreads();
 
-  int cur = 0;
-  for (int l = 1; l < 2 * nb; ++l) {
+  // one layer: weights from the LDS stage, rows software-pipelined; EPI is
+  // called with (row, accumulator)
+  auto run_layer = [&](int l, const char* in, auto&& epi) {
     float wf[(MODE == QTX_CONV_FP32) ? KSW : 1];
     v8bf wh[(MODE == QTX_CONV_FP32) ? 1 : KSW];
     v8bf wl[(MODE == QTX_CONV_FP32) ? 1 : KSW];
-    const char* fr = packed + off_frag + (int64_t)l * frag_bytes;
+    const char* fr = wst + (l & 1) * RC_WSTAGE;
     if constexpr (MODE == QTX_CONV_FP32) {
 #pragma unroll
       for (int i = 0; i < KSW; ++i) wf[i] = reinterpret_cast<const float*>(fr)[i * WAVE + lane];
     } else {
 #pragma unroll
       for (int i = 0; i < KSW; ++i) {
-        const v8bf* src = reinterpret_cast<const v8bf*>(fr) + ((int64_t)i * WAVE + lane) * 2;
+        const v8bf* src = reinterpret_cast<const v8bf*>(fr) + (i * WAVE + lane) * 2;
         wh[i] = src[0];
         wl[i] = src[1];
       }
     }
     const float* bias = pf + bias_off + (int64_t)l * C;
     const v4f bias4 = v4f{bias[4 * gq], bias[4 * gq + 1], bias[4 * gq + 2], bias[4 * gq + 3]};
-    const char* in = act + cur * BUFB + b * CFGB;
-    char* out = act + (cur ^ 1) * BUFB + b * CFGB;
-    const bool is_conv2 = (l % 2) == 1;
-    const int blk = l / 2;
-    const bool last = (l == 2 * nb - 1);
-    const float scale = last ? rsqrtf((float)(nb + 1)) : (is_conv2 ? rsqrtf((float)(blk + 2)) : 1.0f);
-    float m_w = -INFINITY, s_w = 0.0f;
-
     Frag Fm = load_row(in, y0 - 1);
     Frag F0 = load_row(in, y0);
+    v4f accp = bias4;
 #pragma unroll
-    for (int i = 0; i < RPW; ++i) {
-      const int y = y0 + i;
-      Frag Fp = load_row(in, y + 1);
+    for (int i = 0; i <= RPW; ++i) {
       v4f acc = bias4;
-      if constexpr (MODE == QTX_CONV_FP32) {
+      Frag Fp;
+      if (i < RPW) {
+        Fp = load_row(in, y0 + i + 1);
+        if constexpr (MODE == QTX_CONV_FP32) {
 #pragma unroll
-        for (int dy = 0; dy < 3; ++dy) {
-          const Frag& R = (dy == 0) ? Fm : (dy == 1 ? F0 : Fp);
+          for (int dy = 0; dy < 3; ++dy) {
+            const Frag& R = (dy == 0) ? Fm : (dy == 1 ? F0 : Fp);
 #pragma unroll
-          for (int dx = 0; dx < 3; ++dx)
+            for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
-            for (int jp = 0; jp < 4; ++jp)
-              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(wf[(3 * dy + dx) * 4 + jp], R.f[dx][jp], acc, 0, 0, 0);
-        }
-      } else {
+              for (int jp = 0; jp < 4; ++jp)
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(wf[(3 * dy + dx) * 4 + jp], R.f[dx][jp], acc, 0, 0, 0);
+          }
+        } else {
 #pragma unroll
-        for (int dy = 0; dy < 3; ++dy) {
-          const Frag& R = (dy == 0) ? Fm : (dy == 1 ? F0 : Fp);
+          for (int dy = 0; dy < 3; ++dy) {
+            const Frag& R = (dy == 0) ? Fm : (dy == 1 ? F0 : Fp);
 #pragma unroll
-          for (int pr = 0; pr < 2; ++pr) {
-            const int ks = 2 * dy + pr;
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[ks], R.h[pr], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[ks], R.l[pr], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[ks], R.h[pr], acc, 0, 0, 0);
+            for (int pr = 0; pr < 2; ++pr) {
+              const int ks = 2 * dy + pr;
+              acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[ks], R.h[pr], acc, 0, 0, 0);
+              acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[ks], R.l[pr], acc, 0, 0, 0);
+              acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[ks], R.h[pr], acc, 0, 0, 0);
+            }
           }
         }
       }
-      if (!last) {
+      if (i > 0) epi(i - 1, accp);
+      if (i < RPW) {
+        accp = acc;
+        Fm = F0;
+        F0 = Fp;
+      }
+    }
+  };
+
+  int cur = 0;
+  for (int l = 1; l < 2 * nb - 1; ++l) {
+    stage_weights(l + 1);  // stage (l + 1) & 1 was last read at the start of layer l - 1
+    const char* in = act + cur * BUFB + b * CFGB;
+    char* out = act + (cur ^ 1) * BUFB + b * CFGB;
+    const bool is_conv2 = (l % 2) == 1;
+    const float scale = is_conv2 ? rsqrtf((float)(l / 2 + 2)) : 1.0f;
+    auto layer = [&](auto ge_c, auto c2_c) {
+      constexpr bool GE = decltype(ge_c)::value;
+      constexpr bool C2 = decltype(c2_c)::value;
+      run_layer(l, in, [&](int i, v4f acc) {
         v4f o;
-        if (is_conv2) {
-          v4f xr = acc + res[i];
-          res[i] = xr;
+        if constexpr (C2) {
+          v4f xr = acc + resv[i * WAVE];
+          resv[i * WAVE] = xr;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) o[q] = gelu_f(xr[q] * scale, gelu_exact);
+          for (int q = 0; q < 4; ++q) o[q] = gelu_t<GE>(xr[q] * scale);
         } else {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) o[q] = gelu_f(acc[q], gelu_exact);
+          for (int q = 0; q < 4; ++q) o[q] = gelu_t<GE>(acc[q]);
         }
-        store4(out + y * ROWB, o);
-      } else {
-        float hv[4], m = -INFINITY;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          hv[q] = (acc[q] + res[i][q]) * scale;
-          m = fmaxf(m, hv[q]);
-        }
-        m = wave_max(m);
-        float sacc = 0.0f;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) sacc += __expf(hv[q] - m);
-        sacc = wave_sum(sacc);
-        float mm = fmaxf(m_w, m);
-        s_w = s_w * __expf(m_w - mm) + sacc * __expf(m - mm);
-        m_w = mm;
-      }
-      Fm = F0;
-      F0 = Fp;
-    }
-    if (!last) {
-      __syncthreads();
-      cur ^= 1;
+        store4(out + (y0 + i) * ROWB, o);
+      });
+    };
+    if (gelu_exact) {
+      if (is_conv2) layer(std::true_type{}, std::true_type{});
+      else layer(std::true_type{}, std::false_type{});
     } else {
-      if (lane == 0) {
-        red[wv * 2] = m_w;
-        red[wv * 2 + 1] = s_w;
-      }
-      __syncthreads();
-      if (threadIdx.x < B) {
-        int bb = threadIdx.x;
-        float m = -INFINITY;
-        for (int w = 4 * bb; w < 4 * bb + 4; ++w) m = fmaxf(m, red[w * 2]);
-        double ssum = 0.0;
-        for (int w = 4 * bb; w < 4 * bb + 4; ++w)
-          ssum += (double)red[w * 2 + 1] * exp((double)red[w * 2] - (double)m);
-        lp_out[bb] = (double)m + log(ssum) - log((double)C * 256.0);
-      }
-      __syncthreads();
+      if (is_conv2) layer(std::false_type{}, std::true_type{});
+      else layer(std::false_type{}, std::false_type{});
     }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    cur ^= 1;
+  }
+  // ---- last layer (conv2 of the last block, no bias, no GELU) and the head
+  {
+    const int l = 2 * nb - 1;
+    const char* in = act + cur * BUFB + b * CFGB;
+    const float scale = rsqrtf((float)(nb + 1));
+    float m_w = -INFINITY, s_w = 0.0f;
+    run_layer(l, in, [&](int i, v4f acc) {
+      float hv[4], m = -INFINITY;
+      const v4f r = resv[i * WAVE];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        hv[q] = (acc[q] + r[q]) * scale;
+        m = fmaxf(m, hv[q]);
+      }
+      m = wave_max(m);
+      float sacc = 0.0f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sacc += __expf(hv[q] - m);
+      sacc = wave_sum(sacc);
+      float mm = fmaxf(m_w, m);
+      s_w = s_w * __expf(m_w - mm) + sacc * __expf(m - mm);
+      m_w = mm;
+    });
+    if (lane == 0) {
+      red[wv * 2] = m_w;
+      red[wv * 2 + 1] = s_w;
+    }
+    __syncthreads();
+    if (threadIdx.x < B) {
+      int bb = threadIdx.x;
+      float m = -INFINITY;
+      for (int w = 4 * bb; w < 4 * bb + 4; ++w) m = fmaxf(m, red[w * 2]);
+      double ssum = 0.0;
+      for (int w = 4 * bb; w < 4 * bb + 4; ++w) ssum += (double)red[w * 2 + 1] * exp((double)red[w * 2] - (double)m);
+      lp_out[bb] = (double)m + log(ssum) - log((double)C * 256.0);
+    }
+    __syncthreads();
   }
 }
 
@@ -918,7 +975,7 @@ struct RcArgs {
   do {                                                                                                        \
     if constexpr (ROWS == 2)                                                                                  \
       rc_forward_win<MODE>(a.g, a.nb, a.gelu_exact, a.packed, a.off_l0w, a.off_l0b, a.bias_off, a.off_frag,    \
-                           a.frag_bytes, SP, act, red, LP);                                                   \
+                           a.frag_bytes, SP, act, wst, red, LP);                                              \
     else if constexpr (ROWS == 1)                                                                             \
       rc_forward_rows<MODE, B, MAXT, W>(a.g, a.nb, a.gelu_exact, a.packed, a.off_l0w, a.off_l0b, a.bias_off,   \
                                         a.off_frag, a.frag_bytes, SP, act, red, LP);                          \
@@ -931,6 +988,7 @@ template <int C, int MODE, int B, int NMAX, int MAXT, int W, int ROWS>
 __global__ __launch_bounds__(W * 64, (W == 8 ? 2 : 1)) void rc_logpsi_kernel(RcArgs a, const int8_t* __restrict__ s, int64_t ns,
                                                                double* __restrict__ logabs) {
   __shared__ __attribute__((aligned(16))) char act[2 * B * NMAX * C * 4];
+  __shared__ __attribute__((aligned(16))) char wst[ROWS == 2 ? RC_WIN_LDS : 16];
   __shared__ int8_t sp[B * NMAX];
   __shared__ float red[2 * B * W];
   __shared__ double lp[B];
@@ -951,6 +1009,7 @@ __global__ __launch_bounds__(W * 64, (W == 8 ? 2 : 1)) void rc_metropolis_kernel
     int nsteps, int64_t ns, double reweight, double tie_tol, int8_t* __restrict__ s, double* __restrict__ logabs,
     int32_t* __restrict__ n_accept, int32_t* __restrict__ n_near) {
   __shared__ __attribute__((aligned(16))) char act[2 * B * NMAX * C * 4];
+  __shared__ __attribute__((aligned(16))) char wst[ROWS == 2 ? RC_WIN_LDS : 16];
   __shared__ int8_t cur[B * NMAX];
   __shared__ int8_t prop[B * NMAX];
   __shared__ float red[2 * B * W];
@@ -1044,6 +1103,7 @@ __global__ __launch_bounds__(W * 64, (W == 8 ? 2 : 1)) void rc_eloc_kernel(RcArg
                                                              const double* __restrict__ logabs, int64_t ns,
                                                              double* __restrict__ eloc, int32_t* __restrict__ nconn) {
   __shared__ __attribute__((aligned(16))) char act[2 * B * NMAX * C * 4];
+  __shared__ __attribute__((aligned(16))) char wst[ROWS == 2 ? RC_WIN_LDS : 16];
   __shared__ int8_t s0[NMAX];
   __shared__ int8_t prop[B * NMAX];
   __shared__ float red[2 * B * W];
