@@ -165,7 +165,8 @@ int qtx_resconv_metropolis(void* stream, const qtx_resconv_desc* d, const void* 
                            double* logabs, int32_t* n_accept, int32_t* n_near);
 
 /* Local energy: one full forward per valid connected configuration, never
- * materialised.  logabs = log psi(s) of the samples. */
+ * materialised.  logabs = log psi(s) of the samples.  H->noff <= 4096
+ * (<= 3072 for the 16 x 16, 16-channel network). */
 int qtx_resconv_eloc(void* stream, const qtx_resconv_desc* d, const void* packed, const qtx_ham* H,
                      const int8_t* s, const double* logabs, int64_t ns, double* eloc, int32_t* nconn);
 

@@ -688,8 +688,8 @@ __device__ __attribute__((always_inline)) inline void rc_forward_rows(const RcGe
   }
 }
 
-// Sliding-row-window forward for C = 16 on a 16 x 16 grid.  Workgroup: 8
-// waves, 2 configurations; wave w owns rows 4 (w % 4) .. 4 (w % 4) + 3 of
+// Sliding-row-window forward for C = 16 on a 16 x 16 grid.  Workgroup: 4 B
+// waves, B configurations; wave w owns rows 4 (w % 4) .. 4 (w % 4) + 3 of
 // configuration w / 4 and walks them in order keeping the B fragments of
 // input rows y - 1, y, y + 1 in registers: every fragment depends only on its
 // input row (split-bf16 K-steps 2 dy + pr: pr 0 = taps (dy, 0) + (dy, 1),
@@ -705,8 +705,11 @@ __device__ __attribute__((always_inline)) inline void rc_forward_rows(const RcGe
 // RC_WSTAGE bytes) and are read from LDS at the layer start.  The residual
 // stream x (FP32, 16 bytes per lane and row) lives in wst after the stages,
 // in slots private to each lane, which keeps the pipelined loop in registers.
+// With B = 1 a workgroup needs 72 KB of LDS, so two run on a CU: each SIMD
+// then holds one wave of each, and their per-layer barriers are independent,
+// so one wave's MFMA phase can overlap the other's VALU phase.
 constexpr int RC_WSTAGE = 12288;  // one layer's fragments: bf16x3 6 x 64 x 32 B; FP32 9216 B
-constexpr int RC_WIN_LDS = 2 * RC_WSTAGE + 2 * 16 * 64 * 16;  // weight stages + residual slots
+constexpr int rc_win_lds(int B) { return 2 * RC_WSTAGE + B * 16 * 64 * 16; }  // weight stages + residual slots
 typedef __attribute__((address_space(3))) void rc_lds_void;
 
 template <bool GE>
@@ -714,12 +717,12 @@ __device__ __forceinline__ float gelu_t(float x) {
   return gelu_f(x, GE ? 1 : 0);
 }
 
-template <int MODE>
+template <int MODE, int B>
 __device__ __attribute__((always_inline)) inline void rc_forward_win(
     const RcGeom& g, int nb, int gelu_exact, const char* __restrict__ packed, int64_t off_l0w, int64_t off_l0b,
     int64_t bias_off, int64_t off_frag, int64_t frag_bytes, const int8_t* sp, char* act, char* wst, float* red,
     double* lp_out) {
-  constexpr int C = 16, B = 2, W = 8, L0 = 16, RPW = 4;
+  constexpr int C = 16, W = 4 * B, L0 = 16, RPW = 4;
   constexpr int ROWB = 16 * 64;
   constexpr int CFGB = L0 * ROWB;
   constexpr int BUFB = CFGB * B;
@@ -974,7 +977,7 @@ struct RcArgs {
 #define RC_FWD(SP, LP)                                                                                        \
   do {                                                                                                        \
     if constexpr (ROWS == 2)                                                                                  \
-      rc_forward_win<MODE>(a.g, a.nb, a.gelu_exact, a.packed, a.off_l0w, a.off_l0b, a.bias_off, a.off_frag,    \
+      rc_forward_win<MODE, B>(a.g, a.nb, a.gelu_exact, a.packed, a.off_l0w, a.off_l0b, a.bias_off, a.off_frag, \
                            a.frag_bytes, SP, act, wst, red, LP);                                              \
     else if constexpr (ROWS == 1)                                                                             \
       rc_forward_rows<MODE, B, MAXT, W>(a.g, a.nb, a.gelu_exact, a.packed, a.off_l0w, a.off_l0b, a.bias_off,   \
@@ -985,10 +988,10 @@ struct RcArgs {
   } while (0)
 
 template <int C, int MODE, int B, int NMAX, int MAXT, int W, int ROWS>
-__global__ __launch_bounds__(W * 64, (W == 8 ? 2 : 1)) void rc_logpsi_kernel(RcArgs a, const int8_t* __restrict__ s, int64_t ns,
+__global__ __launch_bounds__(W * 64, ((W == 8 || ROWS == 2) ? 2 : 1)) void rc_logpsi_kernel(RcArgs a, const int8_t* __restrict__ s, int64_t ns,
                                                                double* __restrict__ logabs) {
   __shared__ __attribute__((aligned(16))) char act[2 * B * NMAX * C * 4];
-  __shared__ __attribute__((aligned(16))) char wst[ROWS == 2 ? RC_WIN_LDS : 16];
+  __shared__ __attribute__((aligned(16))) char wst[ROWS == 2 ? rc_win_lds(B) : 16];
   __shared__ int8_t sp[B * NMAX];
   __shared__ float red[2 * B * W];
   __shared__ double lp[B];
@@ -1004,12 +1007,12 @@ __global__ __launch_bounds__(W * 64, (W == 8 ? 2 : 1)) void rc_logpsi_kernel(RcA
 }
 
 template <int C, int MODE, int B, int NMAX, int MAXT, int W, int ROWS>
-__global__ __launch_bounds__(W * 64, (W == 8 ? 2 : 1)) void rc_metropolis_kernel(
+__global__ __launch_bounds__(W * 64, ((W == 8 || ROWS == 2) ? 2 : 1)) void rc_metropolis_kernel(
     RcArgs a, int kind, const int32_t* __restrict__ nbr, int width, int hopping, const uint32_t* __restrict__ words,
     int nsteps, int64_t ns, double reweight, double tie_tol, int8_t* __restrict__ s, double* __restrict__ logabs,
     int32_t* __restrict__ n_accept, int32_t* __restrict__ n_near) {
   __shared__ __attribute__((aligned(16))) char act[2 * B * NMAX * C * 4];
-  __shared__ __attribute__((aligned(16))) char wst[ROWS == 2 ? RC_WIN_LDS : 16];
+  __shared__ __attribute__((aligned(16))) char wst[ROWS == 2 ? rc_win_lds(B) : 16];
   __shared__ int8_t cur[B * NMAX];
   __shared__ int8_t prop[B * NMAX];
   __shared__ float red[2 * B * W];
@@ -1097,18 +1100,19 @@ __global__ __launch_bounds__(W * 64, (W == 8 ? 2 : 1)) void rc_metropolis_kernel
 }
 
 constexpr int RC_MAXCAND = 4096;
+constexpr int RC_MAXCAND_WIN = 3072;  // 16 x 16 window kernels: keeps two workgroups' LDS within a CU
 
 template <int C, int MODE, int B, int NMAX, int MAXT, int W, int ROWS>
-__global__ __launch_bounds__(W * 64, (W == 8 ? 2 : 1)) void rc_eloc_kernel(RcArgs a, qtx_ham H, const int8_t* __restrict__ s,
+__global__ __launch_bounds__(W * 64, ((W == 8 || ROWS == 2) ? 2 : 1)) void rc_eloc_kernel(RcArgs a, qtx_ham H, const int8_t* __restrict__ s,
                                                              const double* __restrict__ logabs, int64_t ns,
                                                              double* __restrict__ eloc, int32_t* __restrict__ nconn) {
   __shared__ __attribute__((aligned(16))) char act[2 * B * NMAX * C * 4];
-  __shared__ __attribute__((aligned(16))) char wst[ROWS == 2 ? RC_WIN_LDS : 16];
+  __shared__ __attribute__((aligned(16))) char wst[ROWS == 2 ? rc_win_lds(B) : 16];
   __shared__ int8_t s0[NMAX];
   __shared__ int8_t prop[B * NMAX];
   __shared__ float red[2 * B * W];
   __shared__ double lp[B];
-  __shared__ int16_t vlist[RC_MAXCAND];
+  __shared__ int16_t vlist[ROWS == 2 ? RC_MAXCAND_WIN : RC_MAXCAND];
   __shared__ int nvalid_s;
   __shared__ double diag_s;
   const int N = a.g.N;
@@ -1397,13 +1401,13 @@ static RcArgs rc_args(const RcLayout& L, const void* packed) {
 #define RC_DISPATCH(KERNEL, NS_PER_BLOCK_OUT, GRID_FROM_B, ...)                                        \
   do {                                                                                                 \
     if (L.win) {                                                                                       \
-      constexpr int B_ = 2;                                                                            \
+      constexpr int B_ = 1;                                                                            \
       NS_PER_BLOCK_OUT = B_;                                                                           \
       dim3 grid_ = GRID_FROM_B;                                                                        \
       if (L.mode == QTX_CONV_FP32)                                                                     \
-        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_FP32, 2, 256, 4, 8, 2>), grid_, dim3(512), 0, st, __VA_ARGS__); \
+        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_FP32, 1, 256, 4, 4, 2>), grid_, dim3(256), 0, st, __VA_ARGS__); \
       else                                                                                             \
-        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_BF16X3, 2, 256, 4, 8, 2>), grid_, dim3(512), 0, st, __VA_ARGS__); \
+        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_BF16X3, 1, 256, 4, 4, 2>), grid_, dim3(256), 0, st, __VA_ARGS__); \
     } else if (L.C == 16 && L.L1 == 16 && L.L0 <= 16) {                                                \
       constexpr int B_ = 2;                                                                            \
       NS_PER_BLOCK_OUT = B_;                                                                           \
@@ -1531,13 +1535,15 @@ int qtx_resconv_eloc(void* stream, const qtx_resconv_desc* d, const void* packed
   if (rc) return rc;
   QTX_REQUIRE(H != nullptr, "null qtx_ham");
   QTX_REQUIRE(H->N == d->L0 * d->L1, "qtx_ham.N != L0 * L1");
-  QTX_REQUIRE(H->noff <= RC_MAXCAND, "too many off-diagonal candidates (max 4096)");
+  RcLayout L = rc_layout(d);
+  QTX_REQUIRE(H->noff <= (L.win ? RC_MAXCAND_WIN : RC_MAXCAND),
+              L.win ? "too many off-diagonal candidates (max 3072 on a 16 x 16 lattice with 16 channels)"
+                    : "too many off-diagonal candidates (max 4096)");
   QTX_REQUIRE(H->noff == 0 || (H->off_a && H->off_b && H->off_ra && H->off_rb && H->off_c), "null table");
   QTX_REQUIRE(H->ndiag == 0 || (H->diag_c && H->diag_a && H->diag_b && H->diag_kind), "null table");
   QTX_REQUIRE(packed && ns >= 0 && (ns == 0 || (s && logabs && eloc)), "bad arguments");
   if (ns == 0) return QTX_OK;
   QTX_REQUIRE(ns <= 0x7fffffff, "ns too large");
-  RcLayout L = rc_layout(d);
   RcArgs a = rc_args(L, packed);
   hipStream_t st = as_stream(stream);
   int per = 1;
