@@ -135,6 +135,52 @@ int qtx_rbm_eloc(void* stream, int dtype, const void* packed, int32_t M, int32_t
 int qtx_rbm_jacobian(void* stream, int dtype, const void* packed, int32_t M, int32_t N, const int8_t* s,
                      int64_t ns, int o_dtype, void* O);
 
+/* ---------------------------------------------------------------- serving
+ * Independent RBM_Dense VMC jobs (different N, M, sample counts, couplings)
+ * advanced together: each stage is one persistent launch per register class
+ * R = ceil(M / 64) in {1, 2, 4, 8, 16} covering every job of that class.  A
+ * job's chains use the single-job code and its own Philox stream, so its
+ * results are the same alone and in any mix.  No counterpart in the
+ * reference, which runs one lattice per process (sites/sites.py:46-51); the
+ * per-job stages are the ones qtx_rbm_* replace (float32 parameters).
+ * jobs: host array, read before the call returns (it is staged through a
+ * pinned per-thread buffer).  ws: qtx_rbm_group_ws_bytes(njobs) bytes of
+ * device memory holding the launch's job table; calls that share ws must be
+ * issued on one stream. */
+typedef struct qtx_rbm_job {
+  const float* packed; /* [Wt (N, M) | b (M)] float32 (qtx_rbm_pack) */
+  int32_t M, N;
+  int32_t kind;        /* QTX_LOCALFLIP | QTX_EXCHANGE */
+  int32_t width;       /* row width of nbr */
+  const int32_t* nbr;  /* (N, width) -1-padded neighbour table (EXCHANGE) */
+  int32_t hopping;     /* species moved by EXCHANGE: +1 or -1 */
+  int32_t nsteps;      /* Metropolis steps in a grouped sweep (0: the job sits out) */
+  uint64_t seed;       /* key of the job's sweep stream (qtx_philox_fill, TAG_SWEEP) */
+  int64_t step0;       /* stream step counter of the sweep's first step */
+  int64_t ns;          /* chains = samples */
+  int8_t* s;           /* (ns, N) in/out */
+  double* logabs;      /* (ns) */
+  int32_t* n_accept;   /* (ns) incremented */
+  int32_t* n_near;     /* (ns) incremented */
+  double reweight;
+  double tie_tol;
+  qtx_ham H;           /* local-energy tables (device pointers) */
+  double* eloc;        /* (ns) out */
+  int32_t* nconn;      /* (ns) out, nullable */
+  float* O;            /* (ns, M N + M) out */
+} qtx_rbm_job;
+
+size_t qtx_rbm_group_ws_bytes(int32_t njobs);
+/* logabs of every job's chains (as qtx_rbm_logpsi). */
+int qtx_rbm_group_logpsi(void* stream, const qtx_rbm_job* jobs, int32_t njobs, void* ws, size_t ws_bytes);
+/* nsteps Metropolis steps per job (as qtx_rbm_metropolis with the words of
+ * qtx_philox_fill(seed, TAG_SWEEP, step0, nsteps, 0, ns) generated in-kernel). */
+int qtx_rbm_group_metropolis(void* stream, const qtx_rbm_job* jobs, int32_t njobs, void* ws, size_t ws_bytes);
+/* E_loc of every job's samples (as qtx_rbm_eloc). */
+int qtx_rbm_group_eloc(void* stream, const qtx_rbm_job* jobs, int32_t njobs, void* ws, size_t ws_bytes);
+/* O rows of every job (as qtx_rbm_jacobian with o_dtype QTX_F32). */
+int qtx_rbm_group_jacobian(void* stream, const qtx_rbm_job* jobs, int32_t njobs, void* ws, size_t ws_bytes);
+
 /* ---------------------------------------------------------------- ResConv
  * ResConv(nblocks, channels, kernel 3) on an L0 x L1 periodic grid
  * (model/conv_nets.py:26-183, nn/conv.py:13-68).  channels 16 (L0 * L1 <= 512)

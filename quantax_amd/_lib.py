@@ -30,6 +30,14 @@ class QtxHam(C.Structure):
     ]
 
 
+class QtxRbmJob(C.Structure):
+    _fields_ = [
+        ("packed", vp), ("M", i32), ("N", i32), ("kind", i32), ("width", i32), ("nbr", vp), ("hopping", i32),
+        ("nsteps", i32), ("seed", u64), ("step0", i64), ("ns", i64), ("s", vp), ("logabs", vp), ("n_accept", vp),
+        ("n_near", vp), ("reweight", f64), ("tie_tol", f64), ("H", QtxHam), ("eloc", vp), ("nconn", vp), ("O", vp),
+    ]
+
+
 class QtxResConvDesc(C.Structure):
     _fields_ = [("nblocks", i32), ("channels", i32), ("L0", i32), ("L1", i32), ("precision", i32),
                 ("gelu_exact", i32)]
@@ -50,6 +58,11 @@ PROTOS = {
                                      vp, vp, vp, vp]),
     "qtx_rbm_eloc": (C.c_int, [vp, C.c_int, vp, i32, i32, C.POINTER(QtxHam), vp, i64, vp, vp]),
     "qtx_rbm_jacobian": (C.c_int, [vp, C.c_int, vp, i32, i32, vp, i64, C.c_int, vp]),
+    "qtx_rbm_group_ws_bytes": (sz, [i32]),
+    "qtx_rbm_group_logpsi": (C.c_int, [vp, C.POINTER(QtxRbmJob), i32, vp, sz]),
+    "qtx_rbm_group_metropolis": (C.c_int, [vp, C.POINTER(QtxRbmJob), i32, vp, sz]),
+    "qtx_rbm_group_eloc": (C.c_int, [vp, C.POINTER(QtxRbmJob), i32, vp, sz]),
+    "qtx_rbm_group_jacobian": (C.c_int, [vp, C.POINTER(QtxRbmJob), i32, vp, sz]),
     "qtx_resconv_nparams": (i64, [C.POINTER(QtxResConvDesc)]),
     "qtx_resconv_packed_bytes": (sz, [C.POINTER(QtxResConvDesc)]),
     "qtx_resconv_pack": (C.c_int, [vp, C.POINTER(QtxResConvDesc), vp, vp]),

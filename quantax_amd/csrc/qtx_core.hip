@@ -12,6 +12,7 @@
 #include <string>
 
 #include "common.hpp"
+#include "rbm_device.hpp"
 
 namespace qtx {
 
@@ -69,14 +70,6 @@ __global__ void init_spins_kernel(uint64_t seed, int64_t nchains, int N, int nup
 }
 
 // ================================================================ operator
-// copy one sample's spins into LDS (wave-cooperative)
-__device__ __forceinline__ void load_spins(int8_t* sl, const int8_t* src, int N, int lane) {
-  for (int i = lane; i < N; i += WAVE) sl[i] = src[i];
-}
-
-constexpr int MAXN = 1024;
-constexpr int WAVES_PER_BLOCK = 4;
-
 __global__ __launch_bounds__(256) void ham_diag_kernel(qtx_ham H, const int8_t* __restrict__ s, int64_t ns,
                                                        double* __restrict__ out) {
   __shared__ int8_t sbuf[WAVES_PER_BLOCK][MAXN];
@@ -161,37 +154,6 @@ __global__ void rbm_pack_kernel(const T* __restrict__ theta, int M, int N, T* __
   }
 }
 
-// h for lane-owned hidden units: h[r] = b[i] + sum_j Wt[j, i] s_j,  i = lane + 64 r
-template <typename T, int R>
-__device__ __forceinline__ void rbm_hidden(const T* __restrict__ Wt, const T* __restrict__ b, int M, int N,
-                                           const int8_t* sl, int lane, T (&h)[R]) {
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    int i = lane + WAVE * r;
-    h[r] = (i < M) ? b[i] : T(0);
-  }
-  for (int j = 0; j < N; ++j) {
-    T sj = (T)sl[j];
-    const T* row = Wt + (int64_t)j * M;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      int i = lane + WAVE * r;
-      if (i < M) h[r] += row[i] * sj;
-    }
-  }
-}
-
-template <typename T, int R>
-__device__ __forceinline__ double rbm_sum_logcosh(const T (&h)[R], int M, int lane) {
-  T acc = T(0);
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    int i = lane + WAVE * r;
-    if (i < M) acc += logcosh(h[r]);
-  }
-  return wave_sum((double)acc);
-}
-
 template <typename T, int R>
 __global__ __launch_bounds__(256) void rbm_logpsi_kernel(const T* __restrict__ packed, int M, int N,
                                                          const int8_t* __restrict__ s, int64_t ns,
@@ -200,22 +162,7 @@ __global__ __launch_bounds__(256) void rbm_logpsi_kernel(const T* __restrict__ p
   int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
   int64_t n = (int64_t)blockIdx.x * WAVES_PER_BLOCK + wv;
   if (n >= ns) return;
-  int8_t* sl = sbuf[wv];
-  load_spins(sl, s + n * N, N, lane);
-  __builtin_amdgcn_wave_barrier();
-  const T* Wt = packed;
-  const T* b = packed + (int64_t)M * N;
-  T h[R];
-  rbm_hidden<T, R>(Wt, b, M, N, sl, lane, h);
-  double la = rbm_sum_logcosh<T, R>(h, M, lane);
-  if (lane == 0) logabs[n] = la;
-  if (h_out) {
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      int i = lane + WAVE * r;
-      if (i < M) h_out[n * M + i] = h[r];
-    }
-  }
+  rbm_chain_logpsi<T, R>(packed, M, N, s + n * N, sbuf[wv], lane, logabs + n, h_out ? h_out + n * M : nullptr);
 }
 
 template <typename T, int R>
@@ -228,67 +175,9 @@ __global__ __launch_bounds__(256) void rbm_metropolis_kernel(
   int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
   int64_t c = (int64_t)blockIdx.x * WAVES_PER_BLOCK + wv;
   if (c >= ns) return;
-  int8_t* sl = sbuf[wv];
-  load_spins(sl, s + c * N, N, lane);
-  __builtin_amdgcn_wave_barrier();
-  const T* Wt = packed;
-  const T* b = packed + (int64_t)M * N;
-  T h[R];
-  rbm_hidden<T, R>(Wt, b, M, N, sl, lane, h);
-  double lp = rbm_sum_logcosh<T, R>(h, M, lane);
-  int n_hop = 0;
-  if (kind == QTX_EXCHANGE) {
-    int cnt = 0;
-    for (int i = lane; i < N; i += WAVE) cnt += (sl[i] == hopping);
-    n_hop = wave_sum(cnt);
-  }
-  int acc_count = 0, near_count = 0;
-  for (int t = 0; t < nsteps; ++t) {
-    uint4 w = reinterpret_cast<const uint4*>(words)[(int64_t)t * ns + c];
-    int fa, fb;
-    bool updated;
-    propose_move(kind, sl, N, nbr, width, hopping, n_hop, w.x, w.y, lane, fa, fb, updated);
-    if (!updated) continue;  // s' == s is never accepted (metropolis.py:314-316)
-    T sa = (T)sl[fa];
-    T sb = (fb >= 0) ? (T)sl[fb] : T(0);
-    const T* ra = Wt + (int64_t)fa * M;
-    const T* rb = Wt + (int64_t)(fb >= 0 ? fb : fa) * M;
-    T hn[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      int i = lane + WAVE * r;
-      T v = h[r];
-      if (i < M) {
-        v -= T(2) * ra[i] * sa;
-        if (fb >= 0) v -= T(2) * rb[i] * sb;
-      }
-      hn[r] = v;
-    }
-    double lpn = rbm_sum_logcosh<T, R>(hn, M, lane);
-    double u = words_to_uniform(w.z, w.w);
-    double thr = log1p(-u);
-    double d = reweight * (lpn - lp);
-    bool accept = (d > thr) || (lp == -INFINITY);
-    near_count += (fabs(d - thr) < tie_tol) ? 1 : 0;
-    if (accept) {
-      acc_count += 1;
-#pragma unroll
-      for (int r = 0; r < R; ++r) h[r] = hn[r];
-      lp = lpn;
-      __builtin_amdgcn_wave_barrier();
-      if (lane == 0) {
-        sl[fa] = -sl[fa];
-        if (fb >= 0) sl[fb] = -sl[fb];
-      }
-      __builtin_amdgcn_wave_barrier();
-    }
-  }
-  for (int i = lane; i < N; i += WAVE) s[c * N + i] = sl[i];
-  if (lane == 0) {
-    logabs[c] = lp;
-    n_accept[c] += acc_count;
-    n_near[c] += near_count;
-  }
+  auto getw = [&](int t) { return reinterpret_cast<const uint4*>(words)[(int64_t)t * ns + c]; };
+  rbm_chain_metropolis<T, R>(packed, M, N, kind, nbr, width, hopping, nsteps, getw, reweight, tie_tol, s + c * N,
+                             sbuf[wv], lane, logabs + c, n_accept + c, n_near + c);
 }
 
 template <typename T, int R>
@@ -299,82 +188,19 @@ __global__ __launch_bounds__(256) void rbm_eloc_kernel(const T* __restrict__ pac
   int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
   int64_t n = (int64_t)blockIdx.x * WAVES_PER_BLOCK + wv;
   if (n >= ns) return;
-  int8_t* sl = sbuf[wv];
-  load_spins(sl, s + n * N, N, lane);
-  __builtin_amdgcn_wave_barrier();
-  const T* Wt = packed;
-  const T* b = packed + (int64_t)M * N;
-  T h[R], lc0[R];
-  rbm_hidden<T, R>(Wt, b, M, N, sl, lane, h);
-#pragma unroll
-  for (int r = 0; r < R; ++r) lc0[r] = (lane + WAVE * r < M) ? logcosh(h[r]) : T(0);
-  double e = diag_energy(H, sl, lane);
-  int cnt = 0;
-  for (int k = 0; k < H.noff; ++k) {
-    if (!cand_valid(H, sl, k)) continue;
-    ++cnt;
-    int fa = H.off_a[k], fb = H.off_b[k];
-    T sa = (T)sl[fa];
-    T sb = (fb >= 0) ? (T)sl[fb] : T(0);
-    const T* ra = Wt + (int64_t)fa * M;
-    const T* rb = Wt + (int64_t)(fb >= 0 ? fb : fa) * M;
-    T dl = T(0);
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      int i = lane + WAVE * r;
-      if (i < M) {
-        T v = h[r] - T(2) * ra[i] * sa;
-        if (fb >= 0) v -= T(2) * rb[i] * sb;
-        dl += logcosh(v) - lc0[r];
-      }
-    }
-    double dlp = wave_sum((double)dl);
-    e += H.off_c[k] * exp(dlp);
-  }
-  if (lane == 0) {
-    eloc[n] = e;
-    if (nconn) nconn[n] = cnt;
-  }
+  rbm_sample_eloc<T, R>(packed, M, N, H, s + n * N, sbuf[wv], lane, eloc + n, nconn ? nconn + n : nullptr);
 }
 
 template <typename T, typename TO>
 __global__ __launch_bounds__(256) void rbm_jacobian_kernel(const T* __restrict__ packed, int M, int N,
                                                            const int8_t* __restrict__ s, int64_t ns,
                                                            TO* __restrict__ O) {
-  // one block per sample: tanh(h) into LDS, then a coalesced write of the row
   __shared__ float tl[MAXN];
   __shared__ double tld[MAXN];
   __shared__ int8_t sl[MAXN];
   int64_t n = blockIdx.x;
   if (n >= ns) return;
-  const T* Wt = packed;
-  const T* b = packed + (int64_t)M * N;
-  for (int j = threadIdx.x; j < N; j += blockDim.x) sl[j] = s[n * N + j];
-  __syncthreads();
-  for (int i = threadIdx.x; i < M; i += blockDim.x) {
-    T h = b[i];
-    for (int j = 0; j < N; ++j) h += Wt[(int64_t)j * M + i] * (T)sl[j];
-    T t = tanh(h);
-    tl[i] = (float)t;
-    tld[i] = (double)t;
-  }
-  __syncthreads();
-  int64_t np = (int64_t)M * N + M;
-  TO* row = O + n * np;
-  for (int64_t k = threadIdx.x; k < np; k += blockDim.x) {
-    TO v;
-    if (k < (int64_t)M * N) {
-      int i = (int)(k / N), j = (int)(k % N);
-      if (sizeof(TO) == 8)
-        v = (TO)(tld[i] * (double)sl[j]);
-      else
-        v = (TO)(tl[i] * (float)sl[j]);
-    } else {
-      int i = (int)(k - (int64_t)M * N);
-      v = (sizeof(TO) == 8) ? (TO)tld[i] : (TO)tl[i];
-    }
-    row[k] = v;
-  }
+  rbm_jacobian_row<T, TO>(packed, M, N, s + n * N, O + n * ((int64_t)M * N + M), tl, tld, sl);
 }
 
 }  // namespace qtx

@@ -651,8 +651,9 @@ __global__ void refine_accum_kernel(const double* __restrict__ Z, int64_t n, int
 }
 
 // ---------------------------------------------------------------- rocBLAS handle
-static std::mutex g_blas_mu;
-static rocblas_handle g_blas[64] = {nullptr};
+// one handle per (thread, device): solves issued from different threads on
+// different streams run concurrently (a rocBLAS handle is not thread-safe)
+static thread_local rocblas_handle g_blas[64] = {nullptr};
 
 static int get_blas(hipStream_t st, rocblas_handle* h) {
   int dev = 0;
@@ -943,7 +944,6 @@ template <typename TI>
 static int minsr_solve_t(hipStream_t st, const TI* O, const double* mean, const double* r, const double* ebar,
                          int64_t ns, int64_t np, int mode, int gram, double rshift, double ashift, double rtol,
                          double atol, double* step, SolveWs& w) {
-  std::lock_guard<std::mutex> lock(g_blas_mu);
   rocblas_handle h;
   int rc = get_blas(st, &h);
   if (rc) return rc;

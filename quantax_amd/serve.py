@@ -1,0 +1,374 @@
+"""Serving loop: independent RBM_Dense VMC jobs on one GPU with continuous batching.
+
+The reference runs one lattice per process (``quantax/sites/sites.py:46-51``)
+and has no serving layer; this module is the request loop around the per-job
+VMC step of ``optimizer/driver.py:30-39`` (sweep, E_loc, Jacobian, MinSR
+solve, update), for a stream of jobs that differ in lattice size, coupling,
+RBM width and sample count.
+
+* Admission: a job is admitted (FIFO) once it has arrived and its device
+  footprint (O, the solver workspace, chains) fits the HBM budget next to the
+  jobs already running; finished jobs release their memory at once.
+* Continuous batching: every round advances every admitted job by one VMC
+  step.  Sampling, E_loc and the Jacobian of all jobs are grouped launches
+  (``qtx_rbm_group_*``: one persistent launch per register class), so a job
+  that arrives while others run joins the next round's launches.  The per-job
+  SR solves (Gram, soft inverse, update) then run on per-job streams from a
+  small thread pool, so their kernels overlap on the device.
+* Each job keeps its own Philox streams and the grouped kernels run the
+  single-job per-chain code, so a job's trajectory is the same alone and in
+  any mix (``test_gpu_serve.py`` checks it).
+* Latency of a job = completion time - arrival time; the report gives p50 and
+  p99 beside the aggregate VMC steps/s.
+
+``run_sequential`` is the baseline the stream is compared with: the same
+jobs one after another through the single-job entry points (``qtx_rbm_*``)
+with the plain FP64 Gram.
+"""
+from __future__ import annotations
+
+import time
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import operator as qop
+from . import sites as qsites
+from .model import RBM_Dense
+from .optimizer import auto_pinv_eig, _GRAM
+
+_SEED_INIT = 0x5DEECE66D  # initial-spin key offset used by the single-job samplers
+
+
+@dataclass
+class JobSpec:
+    """One VMC job: J1-J2 Heisenberg (Marshall sign, Sz = 0) on an L x L
+    periodic square lattice, RBM_Dense with alpha N hidden units (float32,
+    N(0, init_std) weights), ``ns`` samples = chains, ``nsteps`` MinSR steps."""
+    job_id: int
+    L: int
+    J2: float
+    alpha: int
+    ns: int
+    nsteps: int = 50
+    seed: int = 0
+    arrival: float = 0.0
+    lr: float = 1e-2
+    rtol: float = 1e-4
+    init_std: float = 0.01
+
+    @property
+    def N(self) -> int:
+        return self.L * self.L
+
+    @property
+    def M(self) -> int:
+        return self.alpha * self.N
+
+    @property
+    def nparams(self) -> int:
+        return self.M * self.N + self.M
+
+
+def c5_stream(njobs: int = 64, mean_interarrival: float = 0.25, nsteps: int = 50, seed: int = 0,
+              L_choices: Sequence[int] = (6, 8, 10, 12, 14, 16), alpha_choices: Sequence[int] = (1, 2, 4),
+              ns_choices: Sequence[int] = (1024, 2048, 4096, 8192), J2_range=(0.0, 0.6)) -> List[JobSpec]:
+    """The served workload: Poisson arrivals (exponential gaps), L, alpha and
+    the sample count uniform over their choices, J2 uniform in its range."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for j in range(njobs):
+        t += float(rng.exponential(mean_interarrival))
+        out.append(JobSpec(job_id=j, L=int(rng.choice(L_choices)), J2=float(rng.uniform(*J2_range)),
+                           alpha=int(rng.choice(alpha_choices)), ns=int(rng.choice(ns_choices)), nsteps=nsteps,
+                           seed=int(rng.integers(1, 2**62)), arrival=t))
+    return out
+
+
+def footprint_bytes(spec: JobSpec, gram: str) -> int:
+    """Device bytes a job holds while admitted (O, solver workspace, chains)."""
+    ns, npar, N = spec.ns, spec.nparams, spec.N
+    ws = int(_lib.lib().qtx_minsr_ws_bytes(ns, npar, _lib.QTX_SOLVE_EIGH_PINV, _GRAM[gram]))
+    o_bytes = ns * npar * 4
+    small = ns * (N + 8 * 4 + 4 * 2) + npar * (4 * 2 + 8 * 2) + (1 << 20)
+    return o_bytes + ws + small
+
+
+class JobRun:
+    """Device state of one admitted job."""
+
+    def __init__(self, spec: JobSpec, gram: str = "bf16x3"):
+        self.spec = spec
+        self.gram = gram
+        N = spec.N
+        dev = _lib.device()
+        # lattice, Hamiltonian and model are built with this job's lattice current;
+        # afterwards the job only uses its own device copies
+        lat = qsites.Square(spec.L, Nparticles=(N // 2, N - N // 2))
+        self._H = qop.Heisenberg(J=[1.0, spec.J2], n_neighbor=[1, 2], msr=True)
+        self.ham = self._H.device_tables()
+        tab = lat.neighbor_table(1)
+        self.nbr = torch.from_numpy(np.ascontiguousarray(tab)).to(dev)
+        self.width = int(tab.shape[1])
+        self.hopping = 1 if 2 * (N // 2) <= N else -1
+        self.model = RBM_Dense(spec.M, dtype=np.float32, init_std=spec.init_std, seed=spec.seed)
+        ns = spec.ns
+        self.s = torch.empty((ns, N), dtype=torch.int8, device=dev)
+        _lib.call("qtx_init_spins", _lib.stream(), spec.seed ^ _SEED_INIT, ns, N, N // 2, self.s.data_ptr())
+        self.logabs = torch.zeros(ns, dtype=torch.float64, device=dev)
+        self.n_accept = torch.zeros(ns, dtype=torch.int32, device=dev)
+        self.n_near = torch.zeros(ns, dtype=torch.int32, device=dev)
+        self.eloc = torch.empty(ns, dtype=torch.float64, device=dev)
+        self.O = torch.empty((ns, spec.nparams), dtype=torch.float32, device=dev)
+        self.mean = torch.empty(spec.nparams, dtype=torch.float64, device=dev)
+        self.ebar = torch.empty(ns, dtype=torch.float64, device=dev)
+        self.out2 = torch.empty(2, dtype=torch.float64, device=dev)
+        self.step = torch.empty(spec.nparams, dtype=torch.float64, device=dev)
+        self.applied = torch.empty(1, dtype=torch.int32, device=dev)
+        self.energy = torch.full((spec.nsteps,), float("nan"), dtype=torch.float64, device=dev)
+        self.solver = auto_pinv_eig(rtol=spec.rtol, gram=gram)
+        nws = int(_lib.lib().qtx_minsr_ws_bytes(ns, spec.nparams, self.solver.mode, _GRAM[gram]))
+        self.ws = torch.empty(max(nws, 1), dtype=torch.uint8, device=dev)
+        self.sweep_counter = 0       # Philox step counter of the sweep stream
+        self.thermalised = False
+        self.steps_done = 0
+        self.stream = torch.cuda.Stream(device=dev)  # default priority
+        self.done_event = torch.cuda.Event()
+
+    # ------------------------------------------------------------ sampling stage
+    def sweep_steps(self) -> int:
+        """Metropolis steps of this job's next sweep: 2N, plus 20N of
+        thermalisation before the first (metropolis.py:125-132)."""
+        N = self.spec.N
+        return 2 * N + (0 if self.thermalised else 20 * N)
+
+    def job_struct(self, nsteps: int) -> _lib.QtxRbmJob:
+        j = _lib.QtxRbmJob()
+        j.packed = self.model.packed.data_ptr()
+        j.M, j.N = self.spec.M, self.spec.N
+        j.kind = _lib.QTX_EXCHANGE
+        j.width = self.width
+        j.nbr = self.nbr.data_ptr()
+        j.hopping = self.hopping
+        j.nsteps = nsteps
+        j.seed = self.spec.seed
+        j.step0 = self.sweep_counter
+        j.ns = self.spec.ns
+        j.s = self.s.data_ptr()
+        j.logabs = self.logabs.data_ptr()
+        j.n_accept = self.n_accept.data_ptr()
+        j.n_near = self.n_near.data_ptr()
+        j.reweight = 2.0
+        j.tie_tol = 1e-9
+        j.H = self.ham
+        j.eloc = self.eloc.data_ptr()
+        j.nconn = None
+        j.O = self.O.data_ptr()
+        return j
+
+    def advance_counters(self, nsteps: int) -> None:
+        self.sweep_counter += nsteps
+        self.thermalised = True
+
+    def sample_single(self) -> None:
+        """The same sweep, E_loc and Jacobian through the single-job entry points."""
+        ns, N, M = self.spec.ns, self.spec.N, self.spec.M
+        nsteps = self.sweep_steps()
+        words = torch.empty((nsteps, ns, 4), dtype=torch.int32, device=self.s.device)
+        _lib.call("qtx_philox_fill", _lib.stream(), self.spec.seed, _lib.TAG_SWEEP, self.sweep_counter, nsteps, 0, ns,
+                  words.data_ptr())
+        _lib.call("qtx_rbm_metropolis", _lib.stream(), _lib.QTX_F32, self.model.packed.data_ptr(), M, N,
+                  _lib.QTX_EXCHANGE, self.nbr.data_ptr(), self.width, self.hopping, words.data_ptr(), nsteps, ns, 2.0,
+                  1e-9, self.s.data_ptr(), self.logabs.data_ptr(), self.n_accept.data_ptr(), self.n_near.data_ptr())
+        self.advance_counters(nsteps)
+        _lib.call("qtx_rbm_eloc", _lib.stream(), _lib.QTX_F32, self.model.packed.data_ptr(), M, N, self.ham,
+                  self.s.data_ptr(), ns, self.eloc.data_ptr(), None)
+        _lib.call("qtx_rbm_jacobian", _lib.stream(), _lib.QTX_F32, self.model.packed.data_ptr(), M, N,
+                  self.s.data_ptr(), ns, _lib.QTX_F32, self.O.data_ptr())
+
+    # ------------------------------------------------------------ SR stage
+    def sr_update(self) -> None:
+        """Ebar, Omean, MinSR/SR solve and the guarded update on the current
+        stream; the energy of this step goes to ``self.energy``."""
+        sp = self.spec
+        ns, npar = sp.ns, sp.nparams
+        st = _lib.stream()
+        _lib.call("qtx_ebar", st, self.eloc.data_ptr(), None, ns, self.ebar.data_ptr(), self.out2.data_ptr())
+        _lib.call("qtx_colmean", st, _lib.QTX_F32, self.O.data_ptr(), ns, npar, self.mean.data_ptr())
+        s = self.solver
+        _lib.call("qtx_minsr_solve", st, _lib.QTX_F32, self.O.data_ptr(), self.mean.data_ptr(), None,
+                  self.ebar.data_ptr(), ns, npar, s.mode, _GRAM[s.gram], s.rshift, s.ashift, s.rtol, s.atol,
+                  self.step.data_ptr(), self.ws.data_ptr(), self.ws.numel())
+        _lib.call("qtx_apply_update", st, _lib.QTX_F32, self.model.theta.data_ptr(), self.step.data_ptr(), sp.lr,
+                  npar, self.applied.data_ptr())
+        self.model.repack()
+        self.energy[self.steps_done: self.steps_done + 1].copy_(self.out2[0:1])
+        self.steps_done += 1
+
+    def result(self) -> dict:
+        return {"job_id": self.spec.job_id, "energy": self.energy.cpu().numpy(),
+                "theta": self.model.theta.detach().cpu().numpy()}
+
+
+def admit(pending: List[JobSpec], now: float, used: int, budget: int, cost) -> List[JobSpec]:
+    """FIFO admission: pop from the head of ``pending`` (sorted by arrival)
+    every job that has arrived and fits next to ``used`` bytes; stop at the
+    first that does not (no overtaking, so a large job is not starved)."""
+    out = []
+    while pending and pending[0].arrival <= now and used + cost(pending[0]) <= budget:
+        sp = pending.pop(0)
+        used += cost(sp)
+        out.append(sp)
+    return out
+
+
+@dataclass
+class ServeReport:
+    latencies: List[float] = field(default_factory=list)
+    makespan: float = 0.0
+    total_steps: int = 0
+    rounds: int = 0
+    max_active: int = 0
+    results: dict = field(default_factory=dict)
+
+    def summary(self) -> dict:
+        lat = np.asarray(self.latencies)
+        return {"jobs": len(lat), "vmc_steps": self.total_steps, "makespan_s": round(self.makespan, 3),
+                "steps_per_s": round(self.total_steps / self.makespan, 3) if self.makespan > 0 else None,
+                "latency_p50_s": round(float(np.percentile(lat, 50)), 3) if lat.size else None,
+                "latency_p99_s": round(float(np.percentile(lat, 99)), 3) if lat.size else None,
+                "rounds": self.rounds, "max_active": self.max_active}
+
+
+class Server:
+    """Continuous-batching VMC server on the current GPU."""
+
+    def __init__(self, hbm_budget: Optional[int] = None, gram: str = "bf16x3", workers: int = 4,
+                 keep_results: bool = False, progress=None):
+        if hbm_budget is None:
+            free, _ = torch.cuda.mem_get_info()
+            hbm_budget = int(0.7 * free)
+        self.budget = int(hbm_budget)
+        self.gram = gram
+        self.workers = int(workers)
+        self.keep_results = keep_results
+        self.progress = progress
+        self._ws = None
+        self._pool = ThreadPoolExecutor(max_workers=self.workers) if self.workers > 1 else None
+
+    def close(self) -> None:
+        if self._pool is not None:
+            self._pool.shutdown()
+            self._pool = None
+
+    def _group_ws(self, njobs: int) -> torch.Tensor:
+        need = max(int(_lib.lib().qtx_rbm_group_ws_bytes(njobs)), 1)
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = torch.empty(max(need, 64 * 1024), dtype=torch.uint8, device=_lib.device())
+        return self._ws
+
+    def _grouped(self, name: str, jobs: List[JobRun], nsteps: Optional[List[int]] = None) -> None:
+        arr = (_lib.QtxRbmJob * len(jobs))()
+        for i, j in enumerate(jobs):
+            arr[i] = j.job_struct(0 if nsteps is None else nsteps[i])
+        ws = self._group_ws(len(jobs))
+        _lib.call(name, _lib.stream(), arr, len(jobs), ws.data_ptr(), ws.numel())
+
+    def round(self, active: List[JobRun]) -> None:
+        """One VMC step of every active job."""
+        nst = [j.sweep_steps() for j in active]
+        self._grouped("qtx_rbm_group_metropolis", active, nst)
+        for j, n in zip(active, nst):
+            j.advance_counters(n)
+        self._grouped("qtx_rbm_group_eloc", active)
+        self._grouped("qtx_rbm_group_jacobian", active)
+        ready = torch.cuda.Event()
+        ready.record()
+        main = torch.cuda.current_stream()
+
+        def sr(job: JobRun):
+            with torch.cuda.stream(job.stream):
+                job.stream.wait_event(ready)
+                job.sr_update()
+                job.done_event.record(job.stream)
+
+        if self._pool is None:
+            for j in active:
+                sr(j)
+        else:
+            list(self._pool.map(sr, active))
+        for j in active:
+            main.wait_event(j.done_event)
+
+    def run(self, specs: Sequence[JobSpec]) -> ServeReport:
+        rep = ServeReport()
+        pending = sorted(specs, key=lambda s: (s.arrival, s.job_id))
+        for s in pending:
+            if footprint_bytes(s, self.gram) > self.budget:
+                raise ValueError(f"job {s.job_id} needs {footprint_bytes(s, self.gram)} bytes, "
+                                 f"more than the HBM budget {self.budget}")
+        active: List[JobRun] = []
+        used = 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        while pending or active:
+            now = time.perf_counter() - t0
+            for sp in admit(pending, now, used, self.budget, lambda s: footprint_bytes(s, self.gram)):
+                used += footprint_bytes(sp, self.gram)
+                active.append(JobRun(sp, self.gram))
+            if not active:
+                time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
+                continue
+            rep.max_active = max(rep.max_active, len(active))
+            self.round(active)
+            rep.rounds += 1
+            finished = [j for j in active if j.steps_done >= j.spec.nsteps]
+            if finished:
+                torch.cuda.synchronize()
+                tdone = time.perf_counter() - t0
+                for j in finished:
+                    rep.latencies.append(tdone - j.spec.arrival)
+                    rep.total_steps += j.spec.nsteps
+                    used -= footprint_bytes(j.spec, self.gram)
+                    if self.keep_results:
+                        rep.results[j.spec.job_id] = j.result()
+                    active.remove(j)
+                if self.progress:
+                    self.progress(f"{len(rep.latencies)} jobs done, {len(active)} active, {len(pending)} waiting, "
+                                  f"t = {tdone:.1f} s")
+        torch.cuda.synchronize()
+        rep.makespan = time.perf_counter() - t0
+        return rep
+
+
+def run_sequential(specs: Sequence[JobSpec], gram: str = "fp64", nsteps: Optional[int] = None,
+                   keep_results: bool = False, progress=None) -> ServeReport:
+    """The jobs one after another through the single-job entry points (no
+    grouping, no overlap); ``nsteps`` caps the steps per job (the per-step
+    cost of a job does not change over its run)."""
+    rep = ServeReport()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k, sp in enumerate(specs):
+        job = JobRun(sp, gram)
+        n = sp.nsteps if nsteps is None else min(nsteps, sp.nsteps)
+        ts = time.perf_counter()
+        for _ in range(n):
+            job.sample_single()
+            job.sr_update()
+        torch.cuda.synchronize()
+        rep.latencies.append(time.perf_counter() - ts)
+        rep.total_steps += n
+        if keep_results:
+            rep.results[sp.job_id] = job.result()
+        del job
+        if progress and (k + 1) % 8 == 0:
+            progress(f"sequential: {k + 1}/{len(specs)} jobs, t = {time.perf_counter() - t0:.1f} s")
+    torch.cuda.synchronize()
+    rep.makespan = time.perf_counter() - t0
+    return rep
