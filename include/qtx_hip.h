@@ -257,9 +257,12 @@ int qtx_minsr_solve(void* stream, int o_dtype, const void* O, const double* mean
                     double ashift, double rtol, double atol, double* step, void* ws, size_t ws_bytes);
 /* What the calling thread's last qtx_minsr_solve did: path 0 Cholesky shift,
  * 1 partial fractions, 2 dsyevd (-1: none or failed); Lanczos steps and the
- * lambda_max they gave (0 when not run).  Diagnostic only: no counterpart in
- * the reference, which always runs eigh. */
-int qtx_minsr_last_path(int* path, int* lanczos_steps, double* lambda_max);
+ * lambda_max they gave (0 when not run); fallback (nullable): why EIGH_PINV
+ * left the partial-fraction route for dsyevd: 0 it did not, 1 cutoff below
+ * 1e-5 lambda_max, 2 non-finite Lanczos coefficients (non-finite T), 3
+ * lambda_max <= 0, 4 a Cholesky factorisation failed.  Diagnostic only: no
+ * counterpart in the reference, which always runs eigh. */
+int qtx_minsr_last_path(int* path, int* lanczos_steps, double* lambda_max, int* fallback);
 
 /* theta <- theta - lr * step in dtype, skipped entirely when any step entry is
  * non-finite (state/variational.py:558-579).  *applied (device int32) = 1/0. */

@@ -79,7 +79,7 @@ PROTOS = {
                                   vp, sz]),
     "qtx_gram_ws_bytes": (sz, [i64, i64, C.c_int]),
     "qtx_gram": (C.c_int, [vp, C.c_int, vp, vp, vp, i64, i64, C.c_int, vp, vp, sz]),
-    "qtx_minsr_last_path": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(f64)]),
+    "qtx_minsr_last_path": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(f64), C.POINTER(C.c_int)]),
     "qtx_apply_update": (C.c_int, [vp, C.c_int, vp, vp, f64, i64, vp]),
 }
 

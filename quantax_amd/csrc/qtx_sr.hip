@@ -740,6 +740,7 @@ static size_t ws_layout(int64_t ns, int64_t np, int mode, int gram, SolveWs* w, 
 static thread_local int g_last_path = -1;
 static thread_local int g_last_lanczos = 0;
 static thread_local double g_last_lmax = 0.0;
+static thread_local int g_last_fallback = 0;  // why the partial-fraction route was left (0: it was not)
 
 // largest eigenvalue of the m x m symmetric tridiagonal (diag a[0..m), off-diagonal
 // e[1..m) between rows i-1 and i) by Sturm-count bisection
@@ -850,7 +851,10 @@ static int pinv_partial_fractions(hipStream_t st, rocblas_handle h, int64_t n, d
   const double one = 1.0, zero = 0.0;
   const unsigned gb = (unsigned)nblk(n, 256);
   int rc;
-  if (atol == 0.0 && !(rtol >= 1e-5)) return 0;  // cutoff too small for the refinement to contract
+  if (atol == 0.0 && !(rtol >= 1e-5)) {  // cutoff too small for the refinement to contract
+    g_last_fallback = 1;
+    return 0;
+  }
   // lambda_max = max |eigenvalue| of the PSD T by Lanczos (three-term recurrence,
   // convergence of the largest Ritz value checked every 32 steps)
   double* alpha = w.ab;
@@ -877,7 +881,10 @@ static int pinv_partial_fractions(hipStream_t st, rocblas_handle h, int64_t n, d
     int m = done;
     bool exact = false;
     for (int j = 0; j < done; ++j) {
-      if (!std::isfinite(ha[j]) || !std::isfinite(hb[j + 1])) return 0;
+      if (!std::isfinite(ha[j]) || !std::isfinite(hb[j + 1])) {
+        g_last_fallback = 2;
+        return 0;
+      }
       if (hb[j + 1] == 0.0) {  // invariant Krylov subspace: the Ritz values are exact
         m = j + 1;
         exact = true;
@@ -893,9 +900,15 @@ static int pinv_partial_fractions(hipStream_t st, rocblas_handle h, int64_t n, d
   }
   g_last_lanczos = msteps;
   g_last_lmax = lmax;
-  if (!(lmax > 0.0) || !std::isfinite(lmax)) return 0;
+  if (!(lmax > 0.0) || !std::isfinite(lmax)) {
+    g_last_fallback = 3;
+    return 0;
+  }
   const double c = rtol * lmax + atol, rel = c / lmax;
-  if (!(rel >= 1e-5) || !std::isfinite(c)) return 0;
+  if (!(rel >= 1e-5) || !std::isfinite(c)) {
+    g_last_fallback = 1;
+    return 0;
+  }
   // refinement contracts by about 4 u / rel^2 per pass
   const int iters = rel >= 1e-3 ? 2 : (rel >= 1e-4 ? 3 : 4);
   rc = rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, ni, ni, &one, w.T, ni, &zero, w.T2, ni),
@@ -933,7 +946,10 @@ static int pinv_partial_fractions(hipStream_t st, rocblas_handle h, int64_t n, d
   int32_t info_h[4] = {0, 0, 0, 0};
   QTX_HIP(hipMemcpyAsync(info_h, w.info, sizeof(info_h), hipMemcpyDeviceToHost, st));
   QTX_HIP(hipStreamSynchronize(st));
-  if (info_h[0] || info_h[1] || info_h[2]) return 0;
+  if (info_h[0] || info_h[1] || info_h[2]) {
+    g_last_fallback = 4;
+    return 0;
+  }
   hipLaunchKernelGGL(copy_f64_kernel, dim3(gb), dim3(256), 0, st, w.x, w.y, n);
   QTX_LAUNCH_CHECK();
   g_last_path = 1;
@@ -1044,6 +1060,7 @@ int qtx_minsr_solve(void* stream, int o_dtype, const void* O, const double* mean
   g_last_path = -1;
   g_last_lanczos = 0;
   g_last_lmax = 0.0;
+  g_last_fallback = 0;
   hipStream_t st = as_stream(stream);
   if (o_dtype == QTX_F32)
     return minsr_solve_t<float>(st, (const float*)O, mean, r, ebar, ns, np, mode, gram, rshift, ashift, rtol, atol,
@@ -1052,8 +1069,9 @@ int qtx_minsr_solve(void* stream, int o_dtype, const void* O, const double* mean
                                step, w);
 }
 
-int qtx_minsr_last_path(int* path, int* lanczos_steps, double* lambda_max) {
+int qtx_minsr_last_path(int* path, int* lanczos_steps, double* lambda_max, int* fallback) {
   QTX_REQUIRE(path && lanczos_steps && lambda_max, "bad arguments");
+  if (fallback) *fallback = g_last_fallback;
   *path = g_last_path;
   *lanczos_steps = g_last_lanczos;
   *lambda_max = g_last_lmax;

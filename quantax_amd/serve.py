@@ -39,7 +39,7 @@ from . import _lib
 from . import operator as qop
 from . import sites as qsites
 from .model import RBM_Dense
-from .optimizer import auto_pinv_eig, _GRAM
+from .optimizer import auto_pinv_eig, last_solve_info, _GRAM
 
 _SEED_INIT = 0x5DEECE66D  # initial-spin key offset used by the single-job samplers
 
@@ -134,6 +134,7 @@ class JobRun:
         self.solver = auto_pinv_eig(rtol=spec.rtol, gram=gram)
         nws = int(_lib.lib().qtx_minsr_ws_bytes(ns, spec.nparams, self.solver.mode, _GRAM[gram]))
         self.ws = torch.empty(max(nws, 1), dtype=torch.uint8, device=dev)
+        self.solve_paths: List[str] = []  # soft-inverse route taken at each step
         self.sweep_counter = 0       # Philox step counter of the sweep stream
         self.thermalised = False
         self.steps_done = 0
@@ -204,6 +205,8 @@ class JobRun:
         _lib.call("qtx_minsr_solve", st, _lib.QTX_F32, self.O.data_ptr(), self.mean.data_ptr(), None,
                   self.ebar.data_ptr(), ns, npar, s.mode, _GRAM[s.gram], s.rshift, s.ashift, s.rtol, s.atol,
                   self.step.data_ptr(), self.ws.data_ptr(), self.ws.numel())
+        info = last_solve_info()
+        self.solve_paths.append(info["path"] if info["fallback"] is None else f"{info['path']}:{info['fallback']}")
         _lib.call("qtx_apply_update", st, _lib.QTX_F32, self.model.theta.data_ptr(), self.step.data_ptr(), sp.lr,
                   npar, self.applied.data_ptr())
         self.model.repack()
@@ -234,6 +237,7 @@ class ServeReport:
     total_steps: int = 0
     rounds: int = 0
     max_active: int = 0
+    measured_s: Optional[float] = None
     results: dict = field(default_factory=dict)
 
     def summary(self) -> dict:
@@ -242,13 +246,14 @@ class ServeReport:
                 "steps_per_s": round(self.total_steps / self.makespan, 3) if self.makespan > 0 else None,
                 "latency_p50_s": round(float(np.percentile(lat, 50)), 3) if lat.size else None,
                 "latency_p99_s": round(float(np.percentile(lat, 99)), 3) if lat.size else None,
-                "rounds": self.rounds, "max_active": self.max_active}
+                "rounds": self.rounds, "max_active": self.max_active,
+                **({"measured_s": round(self.measured_s, 3)} if self.measured_s is not None else {})}
 
 
 class Server:
     """Continuous-batching VMC server on the current GPU."""
 
-    def __init__(self, hbm_budget: Optional[int] = None, gram: str = "bf16x3", workers: int = 4,
+    def __init__(self, hbm_budget: Optional[int] = None, gram: str = "bf16x3", workers: int = 1,
                  keep_results: bool = False, progress=None):
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info()
@@ -349,26 +354,33 @@ class Server:
 def run_sequential(specs: Sequence[JobSpec], gram: str = "fp64", nsteps: Optional[int] = None,
                    keep_results: bool = False, progress=None) -> ServeReport:
     """The jobs one after another through the single-job entry points (no
-    grouping, no overlap); ``nsteps`` caps the steps per job (the per-step
-    cost of a job does not change over its run)."""
+    grouping, no overlap).  With ``nsteps`` below a job's step count the job
+    runs that many steps and its full run is extrapolated as (first step,
+    which carries the thermalisation) + (steps - 1) x (mean later step): the
+    per-step cost of a job does not change over its run.  ``makespan`` is the
+    extrapolated total, ``measured_s`` what actually ran."""
     rep = ServeReport()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    measured = 0.0
     for k, sp in enumerate(specs):
         job = JobRun(sp, gram)
-        n = sp.nsteps if nsteps is None else min(nsteps, sp.nsteps)
-        ts = time.perf_counter()
+        n = sp.nsteps if nsteps is None else max(2, min(nsteps, sp.nsteps))
+        times = []
         for _ in range(n):
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
             job.sample_single()
             job.sr_update()
-        torch.cuda.synchronize()
-        rep.latencies.append(time.perf_counter() - ts)
-        rep.total_steps += n
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - ts)
+        measured += sum(times)
+        full = times[0] + (sp.nsteps - 1) * float(np.mean(times[1:])) if n < sp.nsteps else sum(times)
+        rep.latencies.append(full)
+        rep.makespan += full
+        rep.total_steps += sp.nsteps
         if keep_results:
             rep.results[sp.job_id] = job.result()
         del job
         if progress and (k + 1) % 8 == 0:
-            progress(f"sequential: {k + 1}/{len(specs)} jobs, t = {time.perf_counter() - t0:.1f} s")
-    torch.cuda.synchronize()
-    rep.makespan = time.perf_counter() - t0
+            progress(f"sequential: {k + 1}/{len(specs)} jobs, {measured:.1f} s measured")
+    rep.measured_s = measured
     return rep
