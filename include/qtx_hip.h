@@ -255,6 +255,34 @@ int qtx_gram(void* stream, int o_dtype, const void* O, const double* mean, const
 int qtx_minsr_solve(void* stream, int o_dtype, const void* O, const double* mean, const double* r,
                     const double* ebar, int64_t ns, int64_t np, int mode, int gram, double rshift,
                     double ashift, double rtol, double atol, double* step, void* ws, size_t ws_bytes);
+/* Many independent solves (different shapes, streams and modes) driven from
+ * one host thread: each phase of every task is enqueued on that task's stream
+ * before any task's next phase reads results back, so the tasks overlap on
+ * the device.  Results are those of qtx_minsr_solve on each task.  Returns 0
+ * when every task succeeded, else the first failing task's code (its message
+ * in qtx_last_error); status / path / fallback are set per task.  No
+ * counterpart in the reference (one solve per process step); it serves the
+ * job stream of qtx_rbm_group_*. */
+typedef struct qtx_minsr_task {
+  void* stream;
+  int32_t o_dtype;      /* QTX_F32 | QTX_F64 */
+  int32_t mode;         /* QTX_SOLVE_* */
+  int32_t gram;         /* QTX_GRAM_* */
+  int32_t status;       /* out: 0 or the QTX_E* code of this task */
+  const void* O;        /* (ns, np) */
+  const double* mean;   /* (np) column means */
+  const double* r;      /* (ns) reweight factors, nullable */
+  const double* ebar;   /* (ns) */
+  int64_t ns, np;
+  double rshift, ashift, rtol, atol;
+  double* step;         /* (np) out */
+  void* ws;             /* qtx_minsr_ws_bytes(ns, np, mode, gram) bytes */
+  size_t ws_bytes;
+  int32_t path;         /* out: as qtx_minsr_last_path */
+  int32_t fallback;     /* out: as qtx_minsr_last_path */
+} qtx_minsr_task;
+int qtx_minsr_solve_many(qtx_minsr_task* tasks, int32_t ntasks);
+
 /* What the calling thread's last qtx_minsr_solve did: path 0 Cholesky shift,
  * 1 partial fractions, 2 dsyevd (-1: none or failed); Lanczos steps and the
  * lambda_max they gave (0 when not run); fallback (nullable): why EIGH_PINV

@@ -38,6 +38,14 @@ class QtxRbmJob(C.Structure):
     ]
 
 
+class QtxMinsrTask(C.Structure):
+    _fields_ = [
+        ("stream", vp), ("o_dtype", i32), ("mode", i32), ("gram", i32), ("status", i32), ("O", vp), ("mean", vp),
+        ("r", vp), ("ebar", vp), ("ns", i64), ("np", i64), ("rshift", f64), ("ashift", f64), ("rtol", f64),
+        ("atol", f64), ("step", vp), ("ws", vp), ("ws_bytes", sz), ("path", i32), ("fallback", i32),
+    ]
+
+
 class QtxResConvDesc(C.Structure):
     _fields_ = [("nblocks", i32), ("channels", i32), ("L0", i32), ("L1", i32), ("precision", i32),
                 ("gelu_exact", i32)]
@@ -77,6 +85,7 @@ PROTOS = {
     "qtx_minsr_ws_bytes": (sz, [i64, i64, C.c_int, C.c_int]),
     "qtx_minsr_solve": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, i64, i64, C.c_int, C.c_int, f64, f64, f64, f64, vp,
                                   vp, sz]),
+    "qtx_minsr_solve_many": (C.c_int, [C.POINTER(QtxMinsrTask), i32]),
     "qtx_gram_ws_bytes": (sz, [i64, i64, C.c_int]),
     "qtx_gram": (C.c_int, [vp, C.c_int, vp, vp, vp, i64, i64, C.c_int, vp, vp, sz]),
     "qtx_minsr_last_path": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(f64), C.POINTER(C.c_int)]),

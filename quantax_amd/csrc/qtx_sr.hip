@@ -10,7 +10,6 @@
 #include <rocsolver/rocsolver.h>
 
 #include <cmath>
-#include <mutex>
 #include <string>
 #include <vector>
 
@@ -651,21 +650,42 @@ __global__ void refine_accum_kernel(const double* __restrict__ Z, int64_t n, int
 }
 
 // ---------------------------------------------------------------- rocBLAS handle
-// one handle per (thread, device): solves issued from different threads on
-// different streams run concurrently (a rocBLAS handle is not thread-safe)
-static thread_local rocblas_handle g_blas[64] = {nullptr};
+// One handle per (thread, device, stream).  A handle carries the device
+// workspace rocSOLVER uses for its temporaries, so one handle switched between
+// streams would let two factorisations in flight on different streams share
+// that scratch; a handle bound to a single stream cannot.  The cache keeps at
+// most kMaxHandles per thread and device; the oldest is retired after a
+// device-wide sync.
+constexpr size_t kMaxHandles = 64;
+struct BlasEntry {
+  hipStream_t stream;
+  rocblas_handle handle;
+};
+static thread_local std::vector<BlasEntry> g_blas[64];
 
 static int get_blas(hipStream_t st, rocblas_handle* h) {
   int dev = 0;
   QTX_HIP(hipGetDevice(&dev));
   if (dev < 0 || dev >= 64) return fail(QTX_EHIP, "device index out of range");
-  if (!g_blas[dev]) {
-    if (rocblas_create_handle(&g_blas[dev]) != rocblas_status_success)
-      return fail(QTX_ESOLVER, "rocblas_create_handle failed");
+  std::vector<BlasEntry>& cache = g_blas[dev];
+  for (const BlasEntry& e : cache)
+    if (e.stream == st) {
+      *h = e.handle;
+      return QTX_OK;
+    }
+  if (cache.size() >= kMaxHandles) {
+    QTX_HIP(hipDeviceSynchronize());
+    rocblas_destroy_handle(cache.front().handle);
+    cache.erase(cache.begin());
   }
-  if (rocblas_set_stream(g_blas[dev], st) != rocblas_status_success)
+  rocblas_handle nh = nullptr;
+  if (rocblas_create_handle(&nh) != rocblas_status_success) return fail(QTX_ESOLVER, "rocblas_create_handle failed");
+  if (rocblas_set_stream(nh, st) != rocblas_status_success) {
+    rocblas_destroy_handle(nh);
     return fail(QTX_ESOLVER, "rocblas_set_stream failed");
-  *h = g_blas[dev];
+  }
+  cache.push_back(BlasEntry{st, nh});
+  *h = nh;
   return QTX_OK;
 }
 
@@ -841,173 +861,310 @@ static int pinv_syevd(hipStream_t st, rocblas_handle h, int64_t n, double rtol, 
   return 1;
 }
 
-// y <- g(T) y by the partial-fraction form of the soft inverse (T and y are kept
-// until success).  Returns 1 when done, 0 when the caller should use dsyevd
-// (cutoff below 1e-5 lambda_max, non-finite or non-positive spectrum, or a
-// failed Cholesky factorisation), or an error.
-static int pinv_partial_fractions(hipStream_t st, rocblas_handle h, int64_t n, double rtol, double atol,
-                                  SolveWs& w) {
-  const rocblas_int ni = (rocblas_int)n;
-  const double one = 1.0, zero = 0.0;
-  const unsigned gb = (unsigned)nblk(n, 256);
-  int rc;
-  if (atol == 0.0 && !(rtol >= 1e-5)) {  // cutoff too small for the refinement to contract
-    g_last_fallback = 1;
-    return 0;
-  }
-  // lambda_max = max |eigenvalue| of the PSD T by Lanczos (three-term recurrence,
-  // convergence of the largest Ritz value checked every 32 steps)
-  double* alpha = w.ab;
-  double* beta = w.ab + kLanczosMax;
-  hipLaunchKernelGGL(lanczos_init_kernel, dim3(1), dim3(1024), 0, st, n, w.V, beta);
-  const int mcap = (int)(n < kLanczosMax ? n : kLanczosMax);
-  std::vector<double> ha(kLanczosMax), hb(kLanczosMax + 1);
-  int done = 0, target = mcap < 48 ? mcap : 48, msteps = 0;
+// ---------------------------------------------------------------- phased solve
+// A solve runs as three phases so that many solves on different streams can
+// be driven from one host thread with batched host syncs
+// (qtx_minsr_solve_many): every task's phase p is enqueued before any task's
+// phase p + 1 reads results back.
+//   phase 1: Gram (and for SR the right-hand side Obar^T ebar); CHOL_SHIFT:
+//            shift + potrf + potrs; EIGH_PINV: the first Lanczos steps.
+//   phase 2: EIGH_PINV: read the Lanczos coefficients (sync), extend until the
+//            largest Ritz value settles, then syrk + three (potrf, refinement)
+//            for the partial fractions; dsyevd where that route does not apply.
+//   phase 3: read the factorisation status (sync); dsyevd fallback; the update
+//            vector x = Obar^T y.
+// Only the MFMA Gram and the Lanczos matvecs of different tasks overlap: a
+// task's rocSOLVER factorisations (potrf/potrs/syrk in phase 1 for
+// CHOL_SHIFT, phase 2 for the partial fractions) are waited for before the
+// next task's are issued.  Run concurrently on different streams and
+// handles, those factorisations failed (potrf info != 0) in runs whose
+// every single solve succeeded; one after another they never failed and the
+// stream of solves was also faster.
+struct Task {
+  qtx_minsr_task* io = nullptr;
+  hipStream_t st = nullptr;
+  SolveWs w{};
+  bool minnorm = false;
+  int64_t n = 0;
+  bool failed = false;
+  bool need_syevd = false;  // phase 2 runs the dsyevd route
+  bool pf = false;          // partial-fraction factorisations enqueued
+  int mcap = 0, done = 0;   // Lanczos steps available / enqueued
   double lmax = 0.0;
-  for (;;) {
-    for (int j = done; j < target; ++j) {
-      double* v = w.V + (j % 3) * n;
-      double* vp = w.V + ((j + 2) % 3) * n;
-      double* vn = w.V + ((j + 1) % 3) * n;
-      rc = rb(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &one, w.T, ni, v, 1, &zero, w.w, 1), "rocblas_dgemv");
-      if (rc) return rc;
-      hipLaunchKernelGGL(lanczos_step_kernel, dim3(1), dim3(1024), 0, st, n, j, w.w, v, vp, vn, alpha, beta);
-    }
-    QTX_LAUNCH_CHECK();
-    done = target;
-    QTX_HIP(hipMemcpyAsync(ha.data(), alpha, (size_t)done * 8, hipMemcpyDeviceToHost, st));
-    QTX_HIP(hipMemcpyAsync(hb.data(), beta, (size_t)(done + 1) * 8, hipMemcpyDeviceToHost, st));
-    QTX_HIP(hipStreamSynchronize(st));
-    int m = done;
-    bool exact = false;
-    for (int j = 0; j < done; ++j) {
-      if (!std::isfinite(ha[j]) || !std::isfinite(hb[j + 1])) {
-        g_last_fallback = 2;
-        return 0;
-      }
-      if (hb[j + 1] == 0.0) {  // invariant Krylov subspace: the Ritz values are exact
-        m = j + 1;
-        exact = true;
-        break;
-      }
-    }
-    lmax = tridiag_max_eig(ha.data(), hb.data(), m);
-    msteps = m;
-    if (exact || done >= mcap) break;
-    double prev = tridiag_max_eig(ha.data(), hb.data(), m - 8);
-    if (fabs(lmax - prev) <= 1e-14 * fabs(lmax)) break;
-    target = done + 32 < mcap ? done + 32 : mcap;
-  }
-  g_last_lanczos = msteps;
-  g_last_lmax = lmax;
-  if (!(lmax > 0.0) || !std::isfinite(lmax)) {
-    g_last_fallback = 3;
-    return 0;
-  }
-  const double c = rtol * lmax + atol, rel = c / lmax;
-  if (!(rel >= 1e-5) || !std::isfinite(c)) {
-    g_last_fallback = 1;
-    return 0;
-  }
-  // refinement contracts by about 4 u / rel^2 per pass
-  const int iters = rel >= 1e-3 ? 2 : (rel >= 1e-4 ? 3 : 4);
-  rc = rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, ni, ni, &one, w.T, ni, &zero, w.T2, ni),
-          "rocblas_dsyrk");
-  if (rc) return rc;
-  hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, st, w.info, (int64_t)4, 0);
-  for (int k = 0; k < 3; ++k) {
-    const double th = 3.14159265358979323846 * (2 * k + 1) / 6.0;
-    const double a = (k == 1) ? 0.0 : c * cos(th), b = c * sin(th);
-    hipLaunchKernelGGL(form_quad_kernel, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, st, w.T2, w.T, n,
-                       2.0 * a, c * c, w.A);
-    QTX_LAUNCH_CHECK();
-    rc = rb(rocsolver_dpotrf(h, rocblas_fill_lower, ni, w.A, ni, w.info + k), "rocsolver_dpotrf");
+  int lanczos = 0;
+  int path = -1, fallback = 0;
+};
+
+static int task_fail(Task& t, int rc) {
+  t.failed = true;
+  t.io->status = rc;
+  return rc;
+}
+
+static int lanczos_enqueue(Task& t, rocblas_handle h, int to) {
+  const rocblas_int ni = (rocblas_int)t.n;
+  const double one = 1.0, zero = 0.0;
+  double* alpha = t.w.ab;
+  double* beta = t.w.ab + kLanczosMax;
+  for (int j = t.done; j < to; ++j) {
+    double* v = t.w.V + (j % 3) * t.n;
+    double* vp = t.w.V + ((j + 2) % 3) * t.n;
+    double* vn = t.w.V + ((j + 1) % 3) * t.n;
+    int rc = rb(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &one, t.w.T, ni, v, 1, &zero, t.w.w, 1),
+                "rocblas_dgemv");
     if (rc) return rc;
-    hipLaunchKernelGGL(refine_init_kernel, dim3(gb), dim3(256), 0, st, w.y, n, w.R, w.Z);
-    for (int it = 0; it < iters; ++it) {
-      rc = rb(rocsolver_dpotrs(h, rocblas_fill_lower, ni, 2, w.A, ni, w.R, ni), "rocsolver_dpotrs");
-      if (rc) return rc;
-      rc = rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, ni, 2, ni, &one, w.T, ni, w.R, ni,
-                            &zero, w.TW, ni),
-              "rocblas_dgemm");
-      if (rc) return rc;
-      hipLaunchKernelGGL(refine_update_kernel, dim3(gb), dim3(256), 0, st, w.R, w.TW, n, a, b, w.Z);
-      if (it + 1 < iters) {
-        rc = rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, ni, 2, ni, &one, w.T, ni, w.Z, ni,
-                              &zero, w.TW, ni),
-                "rocblas_dgemm");
-        if (rc) return rc;
-        hipLaunchKernelGGL(refine_residual_kernel, dim3(gb), dim3(256), 0, st, w.y, w.Z, w.TW, n, a, b, w.R);
-      }
-    }
-    hipLaunchKernelGGL(refine_accum_kernel, dim3(gb), dim3(256), 0, st, w.Z, n, k == 0 ? 1 : 0, w.x);
-    QTX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(lanczos_step_kernel, dim3(1), dim3(1024), 0, t.st, t.n, j, t.w.w, v, vp, vn, alpha, beta);
   }
-  int32_t info_h[4] = {0, 0, 0, 0};
-  QTX_HIP(hipMemcpyAsync(info_h, w.info, sizeof(info_h), hipMemcpyDeviceToHost, st));
-  QTX_HIP(hipStreamSynchronize(st));
-  if (info_h[0] || info_h[1] || info_h[2]) {
-    g_last_fallback = 4;
-    return 0;
-  }
-  hipLaunchKernelGGL(copy_f64_kernel, dim3(gb), dim3(256), 0, st, w.x, w.y, n);
   QTX_LAUNCH_CHECK();
-  g_last_path = 1;
-  return 1;
+  t.done = to;
+  return QTX_OK;
 }
 
 template <typename TI>
-static int minsr_solve_t(hipStream_t st, const TI* O, const double* mean, const double* r, const double* ebar,
-                         int64_t ns, int64_t np, int mode, int gram, double rshift, double ashift, double rtol,
-                         double atol, double* step, SolveWs& w) {
-  rocblas_handle h;
-  int rc = get_blas(st, &h);
-  if (rc) return rc;
-  const bool minnorm = ns < np;
-  const int64_t n = minnorm ? ns : np;
-  QTX_REQUIRE(n <= 0x7fffffff, "matrix too large for rocSOLVER");
-  // Gram
-  if (minnorm) {
-    rc = launch_gram_nt<TI>(st, gram, O, mean, r, ns, np, w.T, w.split);
+static int phase1(Task& t, rocblas_handle h) {
+  qtx_minsr_task& io = *t.io;
+  const TI* O = (const TI*)io.O;
+  int rc;
+  if (t.minnorm) {
+    rc = launch_gram_nt<TI>(t.st, io.gram, O, io.mean, io.r, io.ns, io.np, t.w.T, t.w.split);
     if (rc) return rc;
-    hipLaunchKernelGGL(copy_f64_kernel, dim3(nblk(ns, 256)), dim3(256), 0, st, ebar, w.y, ns);
+    hipLaunchKernelGGL(copy_f64_kernel, dim3(nblk(io.ns, 256)), dim3(256), 0, t.st, io.ebar, t.w.y, io.ns);
   } else {
-    dim3 g(nblk(np, 16), nblk(np, 16));
-    hipLaunchKernelGGL(gram_tn_kernel<TI>, g, dim3(256), 0, st, O, mean, r, ns, np, 1.0 / (double)ns, w.T);
+    dim3 g(nblk(io.np, 16), nblk(io.np, 16));
+    hipLaunchKernelGGL(gram_tn_kernel<TI>, g, dim3(256), 0, t.st, O, io.mean, io.r, io.ns, io.np,
+                       1.0 / (double)io.ns, t.w.T);
     QTX_LAUNCH_CHECK();
-    rc = launch_obar_t_y<TI>(st, O, mean, r, ebar, ns, np, w.y);  // F = Obar^T ebar
+    rc = launch_obar_t_y<TI>(t.st, O, io.mean, io.r, io.ebar, io.ns, io.np, t.w.y);  // F = Obar^T ebar
     if (rc) return rc;
   }
-  if (mode == QTX_SOLVE_CHOL_SHIFT) {
-    const rocblas_int ni = (rocblas_int)n;
-    hipLaunchKernelGGL(shift_diag_kernel, dim3(1), dim3(1024), 0, st, w.T, n, rshift, ashift, w.scalars);
+  const rocblas_int ni = (rocblas_int)t.n;
+  if (io.mode == QTX_SOLVE_CHOL_SHIFT) {
+    hipLaunchKernelGGL(shift_diag_kernel, dim3(1), dim3(1024), 0, t.st, t.w.T, t.n, io.rshift, io.ashift,
+                       t.w.scalars);
     QTX_LAUNCH_CHECK();
-    rc = rb(rocsolver_dpotrf(h, rocblas_fill_lower, ni, w.T, ni, w.info), "rocsolver_dpotrf");
+    rc = rb(rocsolver_dpotrf(h, rocblas_fill_lower, ni, t.w.T, ni, t.w.info), "rocsolver_dpotrf");
     if (rc) return rc;
-    rc = rb(rocsolver_dpotrs(h, rocblas_fill_lower, ni, 1, w.T, ni, w.y, ni), "rocsolver_dpotrs");
-    if (rc) return rc;
-    int32_t info_h = 0;
-    QTX_HIP(hipMemcpyAsync(&info_h, w.info, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    QTX_HIP(hipStreamSynchronize(st));
-    if (info_h != 0) return fail(QTX_ESOLVER, "potrf info = " + std::to_string(info_h));
-    g_last_path = 0;
-  } else {
-    rocblas_pointer_mode pm;
-    rocblas_get_pointer_mode(h, &pm);
-    rocblas_set_pointer_mode(h, rocblas_pointer_mode_host);
-    int done = 0;
-    if (mode == QTX_SOLVE_EIGH_PINV) done = pinv_partial_fractions(st, h, n, rtol, atol, w);
-    if (done == 0) done = pinv_syevd(st, h, n, rtol, atol, w);
-    rocblas_set_pointer_mode(h, pm);
-    if (done < 0) return done;
+    return rb(rocsolver_dpotrs(h, rocblas_fill_lower, ni, 1, t.w.T, ni, t.w.y, ni), "rocsolver_dpotrs");
   }
-  if (minnorm) {
-    rc = launch_obar_t_y<TI>(st, O, mean, r, w.y, ns, np, step);
+  if (io.mode == QTX_SOLVE_EIGH_PINV_DIRECT) {
+    t.need_syevd = true;
+    return QTX_OK;
+  }
+  if (io.atol == 0.0 && !(io.rtol >= 1e-5)) {  // cutoff too small for the refinement to contract
+    t.fallback = 1;
+    t.need_syevd = true;
+    return QTX_OK;
+  }
+  // lambda_max = max |eigenvalue| of the PSD T by Lanczos (three-term
+  // recurrence; the largest Ritz value is checked every 32 steps)
+  hipLaunchKernelGGL(lanczos_init_kernel, dim3(1), dim3(1024), 0, t.st, t.n, t.w.V, t.w.ab + kLanczosMax);
+  t.mcap = (int)(t.n < kLanczosMax ? t.n : kLanczosMax);
+  return lanczos_enqueue(t, h, t.mcap < 48 ? t.mcap : 48);
+}
+
+static int phase2(Task& t, rocblas_handle h) {
+  qtx_minsr_task& io = *t.io;
+  if (io.mode == QTX_SOLVE_CHOL_SHIFT) return QTX_OK;
+  if (!t.need_syevd) {
+    std::vector<double> ha(kLanczosMax), hb(kLanczosMax + 1);
+    double lmax = 0.0;
+    int msteps = 0;
+    bool bad = false;
+    for (;;) {
+      QTX_HIP(hipMemcpyAsync(ha.data(), t.w.ab, (size_t)t.done * 8, hipMemcpyDeviceToHost, t.st));
+      QTX_HIP(hipMemcpyAsync(hb.data(), t.w.ab + kLanczosMax, (size_t)(t.done + 1) * 8, hipMemcpyDeviceToHost, t.st));
+      QTX_HIP(hipStreamSynchronize(t.st));
+      int m = t.done;
+      bool exact = false;
+      for (int j = 0; j < t.done; ++j) {
+        if (!std::isfinite(ha[j]) || !std::isfinite(hb[j + 1])) {
+          bad = true;
+          break;
+        }
+        if (hb[j + 1] == 0.0) {  // invariant Krylov subspace: the Ritz values are exact
+          m = j + 1;
+          exact = true;
+          break;
+        }
+      }
+      if (bad) break;
+      lmax = tridiag_max_eig(ha.data(), hb.data(), m);
+      msteps = m;
+      if (exact || t.done >= t.mcap) break;
+      double prev = tridiag_max_eig(ha.data(), hb.data(), m - 8);
+      if (fabs(lmax - prev) <= 1e-14 * fabs(lmax)) break;
+      int rc = lanczos_enqueue(t, h, t.done + 32 < t.mcap ? t.done + 32 : t.mcap);
+      if (rc) return rc;
+    }
+    t.lanczos = msteps;
+    t.lmax = lmax;
+    const double c = io.rtol * lmax + io.atol, rel = c / lmax;
+    if (bad) {
+      t.fallback = 2;
+    } else if (!(lmax > 0.0) || !std::isfinite(lmax)) {
+      t.fallback = 3;
+    } else if (!(rel >= 1e-5) || !std::isfinite(c)) {
+      t.fallback = 1;
+    }
+    if (t.fallback) {
+      t.need_syevd = true;
+    } else {
+      // refinement contracts by about 4 u / rel^2 per pass
+      const int iters = rel >= 1e-3 ? 2 : (rel >= 1e-4 ? 3 : 4);
+      const rocblas_int ni = (rocblas_int)t.n;
+      const double one = 1.0, zero = 0.0;
+      const unsigned gb = (unsigned)nblk(t.n, 256);
+      int rc = rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, ni, ni, &one, t.w.T, ni, &zero,
+                                t.w.T2, ni),
+                  "rocblas_dsyrk");
+      if (rc) return rc;
+      hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, t.st, t.w.info, (int64_t)4, 0);
+      for (int k = 0; k < 3; ++k) {
+        const double th = 3.14159265358979323846 * (2 * k + 1) / 6.0;
+        const double a = (k == 1) ? 0.0 : c * cos(th), b = c * sin(th);
+        hipLaunchKernelGGL(form_quad_kernel, dim3((unsigned)((t.n * t.n + 255) / 256)), dim3(256), 0, t.st, t.w.T2,
+                           t.w.T, t.n, 2.0 * a, c * c, t.w.A);
+        QTX_LAUNCH_CHECK();
+        rc = rb(rocsolver_dpotrf(h, rocblas_fill_lower, ni, t.w.A, ni, t.w.info + k), "rocsolver_dpotrf");
+        if (rc) return rc;
+        hipLaunchKernelGGL(refine_init_kernel, dim3(gb), dim3(256), 0, t.st, t.w.y, t.n, t.w.R, t.w.Z);
+        for (int it = 0; it < iters; ++it) {
+          rc = rb(rocsolver_dpotrs(h, rocblas_fill_lower, ni, 2, t.w.A, ni, t.w.R, ni), "rocsolver_dpotrs");
+          if (rc) return rc;
+          rc = rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, ni, 2, ni, &one, t.w.T, ni, t.w.R,
+                                ni, &zero, t.w.TW, ni),
+                  "rocblas_dgemm");
+          if (rc) return rc;
+          hipLaunchKernelGGL(refine_update_kernel, dim3(gb), dim3(256), 0, t.st, t.w.R, t.w.TW, t.n, a, b, t.w.Z);
+          if (it + 1 < iters) {
+            rc = rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, ni, 2, ni, &one, t.w.T, ni,
+                                  t.w.Z, ni, &zero, t.w.TW, ni),
+                    "rocblas_dgemm");
+            if (rc) return rc;
+            hipLaunchKernelGGL(refine_residual_kernel, dim3(gb), dim3(256), 0, t.st, t.w.y, t.w.Z, t.w.TW, t.n, a, b,
+                               t.w.R);
+          }
+        }
+        hipLaunchKernelGGL(refine_accum_kernel, dim3(gb), dim3(256), 0, t.st, t.w.Z, t.n, k == 0 ? 1 : 0, t.w.x);
+        QTX_LAUNCH_CHECK();
+      }
+      t.pf = true;
+    }
+  }
+  if (t.need_syevd) {
+    int rc = pinv_syevd(t.st, h, t.n, io.rtol, io.atol, t.w);
+    if (rc < 0) return rc;
+    t.path = 2;
+  }
+  return QTX_OK;
+}
+
+template <typename TI>
+static int phase3(Task& t, rocblas_handle h) {
+  qtx_minsr_task& io = *t.io;
+  if (io.mode == QTX_SOLVE_CHOL_SHIFT || t.pf) {
+    int32_t info_h[4] = {0, 0, 0, 0};
+    QTX_HIP(hipMemcpyAsync(info_h, t.w.info, sizeof(info_h), hipMemcpyDeviceToHost, t.st));
+    QTX_HIP(hipStreamSynchronize(t.st));
+    if (io.mode == QTX_SOLVE_CHOL_SHIFT) {
+      if (info_h[0] != 0) return fail(QTX_ESOLVER, "potrf info = " + std::to_string(info_h[0]));
+      t.path = 0;
+    } else if (info_h[0] || info_h[1] || info_h[2]) {  // T and y are intact: take the dsyevd route
+      t.fallback = 4;
+      int rc = pinv_syevd(t.st, h, t.n, io.rtol, io.atol, t.w);
+      if (rc < 0) return rc;
+      t.path = 2;
+    } else {
+      hipLaunchKernelGGL(copy_f64_kernel, dim3(nblk(t.n, 256)), dim3(256), 0, t.st, t.w.x, t.w.y, t.n);
+      QTX_LAUNCH_CHECK();
+      t.path = 1;
+    }
+  }
+  if (t.minnorm) {
+    int rc = launch_obar_t_y<TI>(t.st, (const TI*)io.O, io.mean, io.r, t.w.y, io.ns, io.np, io.step);
     if (rc) return rc;
   } else {
-    hipLaunchKernelGGL(copy_f64_kernel, dim3(nblk(np, 256)), dim3(256), 0, st, w.y, step, np);
+    hipLaunchKernelGGL(copy_f64_kernel, dim3(nblk(io.np, 256)), dim3(256), 0, t.st, t.w.y, io.step, io.np);
     QTX_LAUNCH_CHECK();
   }
+  return QTX_OK;
+}
+
+static int check_task(const qtx_minsr_task& io) {
+  QTX_REQUIRE(io.o_dtype == QTX_F32 || io.o_dtype == QTX_F64, "o_dtype must be QTX_F32 or QTX_F64");
+  QTX_REQUIRE(io.ns > 0 && io.np > 0 && io.O && io.mean && io.ebar && io.step, "bad arguments");
+  QTX_REQUIRE(io.mode == QTX_SOLVE_CHOL_SHIFT || io.mode == QTX_SOLVE_EIGH_PINV ||
+                  io.mode == QTX_SOLVE_EIGH_PINV_DIRECT,
+              "unknown solver mode");
+  QTX_REQUIRE(io.gram == QTX_GRAM_FP64 || io.gram == QTX_GRAM_FP32 || io.gram == QTX_GRAM_BF16X3,
+              "unknown gram mode");
+  QTX_REQUIRE((io.ns < io.np ? io.ns : io.np) <= 0x7fffffff, "matrix too large for rocSOLVER");
+  size_t need = ws_layout(io.ns, io.np, io.mode, io.gram, nullptr, nullptr);
+  if (io.ws == nullptr || io.ws_bytes < need)
+    return fail(QTX_EWORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+  return QTX_OK;
+}
+
+static int solve_many(qtx_minsr_task* tasks, int ntasks) {
+  std::vector<Task> ts((size_t)ntasks);
+  int first_rc = QTX_OK;
+  std::string first_msg;
+  auto note = [&](Task& t, int rc) {
+    task_fail(t, rc);
+    if (first_rc == QTX_OK) {
+      first_rc = rc;
+      first_msg = qtx_last_error();
+    }
+  };
+  for (int i = 0; i < ntasks; ++i) {
+    Task& t = ts[i];
+    t.io = &tasks[i];
+    t.io->status = QTX_OK;
+    t.io->path = -1;
+    t.io->fallback = 0;
+    int rc = check_task(tasks[i]);
+    if (rc) {
+      note(t, rc);
+      continue;
+    }
+    t.st = as_stream(tasks[i].stream);
+    t.minnorm = tasks[i].ns < tasks[i].np;
+    t.n = t.minnorm ? tasks[i].ns : tasks[i].np;
+    ws_layout(tasks[i].ns, tasks[i].np, tasks[i].mode, tasks[i].gram, &t.w, (char*)tasks[i].ws);
+  }
+  rocblas_handle h = nullptr;
+  for (int phase = 1; phase <= 3; ++phase) {
+    for (Task& t : ts) {
+      if (t.failed) continue;
+      int rc = get_blas(t.st, &h);
+      if (rc == QTX_OK) {
+        rocblas_pointer_mode pm;
+        rocblas_get_pointer_mode(h, &pm);
+        rocblas_set_pointer_mode(h, rocblas_pointer_mode_host);
+        const bool f32 = t.io->o_dtype == QTX_F32;
+        if (phase == 1) rc = f32 ? phase1<float>(t, h) : phase1<double>(t, h);
+        else if (phase == 2) rc = phase2(t, h);
+        else rc = f32 ? phase3<float>(t, h) : phase3<double>(t, h);
+        // factorisations of one task finish before the next task's start
+        const bool factored = (phase == 1 && t.io->mode == QTX_SOLVE_CHOL_SHIFT) || (phase == 2 && t.pf);
+        if (rc == QTX_OK && factored && ntasks > 1) rc = hip_check(hipStreamSynchronize(t.st), "hipStreamSynchronize");
+        rocblas_set_pointer_mode(h, pm);
+      }
+      if (rc) note(t, rc);
+    }
+  }
+  for (Task& t : ts) {
+    t.io->path = t.path;
+    t.io->fallback = t.fallback;
+  }
+  if (ntasks > 0) {
+    const Task& t = ts[ntasks - 1];
+    g_last_path = t.failed ? -1 : t.path;
+    g_last_lanczos = t.lanczos;
+    g_last_lmax = t.lmax;
+    g_last_fallback = t.fallback;
+  }
+  if (first_rc != QTX_OK) return fail(first_rc, first_msg);
   return QTX_OK;
 }
 
@@ -1047,26 +1204,34 @@ size_t qtx_minsr_ws_bytes(int64_t ns, int64_t np, int mode, int gram) {
 int qtx_minsr_solve(void* stream, int o_dtype, const void* O, const double* mean, const double* r,
                     const double* ebar, int64_t ns, int64_t np, int mode, int gram, double rshift, double ashift,
                     double rtol, double atol, double* step, void* ws, size_t ws_bytes) {
-  QTX_REQUIRE(o_dtype == QTX_F32 || o_dtype == QTX_F64, "o_dtype must be QTX_F32 or QTX_F64");
-  QTX_REQUIRE(ns > 0 && np > 0 && O && mean && ebar && step, "bad arguments");
-  QTX_REQUIRE(mode == QTX_SOLVE_CHOL_SHIFT || mode == QTX_SOLVE_EIGH_PINV || mode == QTX_SOLVE_EIGH_PINV_DIRECT,
-              "unknown solver mode");
-  QTX_REQUIRE(gram == QTX_GRAM_FP64 || gram == QTX_GRAM_FP32 || gram == QTX_GRAM_BF16X3, "unknown gram mode");
-  size_t need = qtx_minsr_ws_bytes(ns, np, mode, gram);
-  if (ws == nullptr || ws_bytes < need)
-    return fail(QTX_EWORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
-  SolveWs w;
-  ws_layout(ns, np, mode, gram, &w, (char*)ws);
+  qtx_minsr_task task{};
+  task.stream = stream;
+  task.o_dtype = o_dtype;
+  task.mode = mode;
+  task.gram = gram;
+  task.O = O;
+  task.mean = mean;
+  task.r = r;
+  task.ebar = ebar;
+  task.ns = ns;
+  task.np = np;
+  task.rshift = rshift;
+  task.ashift = ashift;
+  task.rtol = rtol;
+  task.atol = atol;
+  task.step = step;
+  task.ws = ws;
+  task.ws_bytes = ws_bytes;
   g_last_path = -1;
   g_last_lanczos = 0;
   g_last_lmax = 0.0;
   g_last_fallback = 0;
-  hipStream_t st = as_stream(stream);
-  if (o_dtype == QTX_F32)
-    return minsr_solve_t<float>(st, (const float*)O, mean, r, ebar, ns, np, mode, gram, rshift, ashift, rtol, atol,
-                                step, w);
-  return minsr_solve_t<double>(st, (const double*)O, mean, r, ebar, ns, np, mode, gram, rshift, ashift, rtol, atol,
-                               step, w);
+  return solve_many(&task, 1);
+}
+
+int qtx_minsr_solve_many(qtx_minsr_task* tasks, int32_t ntasks) {
+  QTX_REQUIRE(ntasks >= 0 && (ntasks == 0 || tasks != nullptr), "bad task array");
+  return solve_many(tasks, ntasks);
 }
 
 int qtx_minsr_last_path(int* path, int* lanczos_steps, double* lambda_max, int* fallback) {

@@ -13,8 +13,10 @@ RBM width and sample count.
   step.  Sampling, E_loc and the Jacobian of all jobs are grouped launches
   (``qtx_rbm_group_*``: one persistent launch per register class), so a job
   that arrives while others run joins the next round's launches.  The per-job
-  SR solves (Gram, soft inverse, update) then run on per-job streams from a
-  small thread pool, so their kernels overlap on the device.
+  SR solves (Gram, soft inverse, update) then run on per-job streams, all
+  driven from one host thread by ``qtx_minsr_solve_many``, which enqueues
+  each solve phase for every job before it reads any result back, so the
+  jobs' solver kernels overlap on the device.
 * Each job keeps its own Philox streams and the grouped kernels run the
   single-job per-chain code, so a job's trajectory is the same alone and in
   any mix (``test_gpu_serve.py`` checks it).
@@ -28,7 +30,6 @@ with the plain FP64 Gram.
 from __future__ import annotations
 
 import time
-from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence
 
@@ -42,6 +43,14 @@ from .model import RBM_Dense
 from .optimizer import auto_pinv_eig, last_solve_info, _GRAM
 
 _SEED_INIT = 0x5DEECE66D  # initial-spin key offset used by the single-job samplers
+_NSTREAMS = 8  # job solves run on a fixed pool of streams (job id mod 8)
+_streams: List[torch.cuda.Stream] = []
+
+
+def _job_stream(job_id: int) -> torch.cuda.Stream:
+    if not _streams:
+        _streams.extend(torch.cuda.Stream(device=_lib.device()) for _ in range(_NSTREAMS))  # default priority
+    return _streams[job_id % _NSTREAMS]
 
 
 @dataclass
@@ -138,7 +147,7 @@ class JobRun:
         self.sweep_counter = 0       # Philox step counter of the sweep stream
         self.thermalised = False
         self.steps_done = 0
-        self.stream = torch.cuda.Stream(device=dev)  # default priority
+        self.stream = _job_stream(spec.job_id)
         self.done_event = torch.cuda.Event()
 
     # ------------------------------------------------------------ sampling stage
@@ -193,25 +202,44 @@ class JobRun:
                   self.s.data_ptr(), ns, _lib.QTX_F32, self.O.data_ptr())
 
     # ------------------------------------------------------------ SR stage
-    def sr_update(self) -> None:
-        """Ebar, Omean, MinSR/SR solve and the guarded update on the current
-        stream; the energy of this step goes to ``self.energy``."""
+    def sr_pre(self) -> None:
+        """Ebar and Omean on the current stream."""
+        sp, st = self.spec, _lib.stream()
+        _lib.call("qtx_ebar", st, self.eloc.data_ptr(), None, sp.ns, self.ebar.data_ptr(), self.out2.data_ptr())
+        _lib.call("qtx_colmean", st, _lib.QTX_F32, self.O.data_ptr(), sp.ns, sp.nparams, self.mean.data_ptr())
+
+    def sr_task(self) -> _lib.QtxMinsrTask:
+        """The MinSR/SR solve of this step as a qtx_minsr_solve_many task on ``self.stream``."""
+        sp, s = self.spec, self.solver
+        t = _lib.QtxMinsrTask()
+        t.stream = self.stream.cuda_stream
+        t.o_dtype, t.mode, t.gram = _lib.QTX_F32, s.mode, _GRAM[s.gram]
+        t.O, t.mean, t.r, t.ebar = self.O.data_ptr(), self.mean.data_ptr(), None, self.ebar.data_ptr()
+        t.ns, t.np = sp.ns, sp.nparams
+        t.rshift, t.ashift, t.rtol, t.atol = s.rshift, s.ashift, s.rtol, s.atol
+        t.step, t.ws, t.ws_bytes = self.step.data_ptr(), self.ws.data_ptr(), self.ws.numel()
+        return t
+
+    def sr_post(self, path: str) -> None:
+        """The guarded update and this step's energy, on the current stream."""
         sp = self.spec
-        ns, npar = sp.ns, sp.nparams
-        st = _lib.stream()
-        _lib.call("qtx_ebar", st, self.eloc.data_ptr(), None, ns, self.ebar.data_ptr(), self.out2.data_ptr())
-        _lib.call("qtx_colmean", st, _lib.QTX_F32, self.O.data_ptr(), ns, npar, self.mean.data_ptr())
-        s = self.solver
-        _lib.call("qtx_minsr_solve", st, _lib.QTX_F32, self.O.data_ptr(), self.mean.data_ptr(), None,
-                  self.ebar.data_ptr(), ns, npar, s.mode, _GRAM[s.gram], s.rshift, s.ashift, s.rtol, s.atol,
-                  self.step.data_ptr(), self.ws.data_ptr(), self.ws.numel())
-        info = last_solve_info()
-        self.solve_paths.append(info["path"] if info["fallback"] is None else f"{info['path']}:{info['fallback']}")
-        _lib.call("qtx_apply_update", st, _lib.QTX_F32, self.model.theta.data_ptr(), self.step.data_ptr(), sp.lr,
-                  npar, self.applied.data_ptr())
+        self.solve_paths.append(path)
+        _lib.call("qtx_apply_update", _lib.stream(), _lib.QTX_F32, self.model.theta.data_ptr(), self.step.data_ptr(),
+                  sp.lr, sp.nparams, self.applied.data_ptr())
         self.model.repack()
         self.energy[self.steps_done: self.steps_done + 1].copy_(self.out2[0:1])
         self.steps_done += 1
+
+    def sr_update(self) -> None:
+        """Ebar, Omean, MinSR/SR solve and the guarded update on the current
+        stream (the single-job route)."""
+        sp, s = self.spec, self.solver
+        self.sr_pre()
+        _lib.call("qtx_minsr_solve", _lib.stream(), _lib.QTX_F32, self.O.data_ptr(), self.mean.data_ptr(), None,
+                  self.ebar.data_ptr(), sp.ns, sp.nparams, s.mode, _GRAM[s.gram], s.rshift, s.ashift, s.rtol, s.atol,
+                  self.step.data_ptr(), self.ws.data_ptr(), self.ws.numel())
+        info = last_solve_info()
+        self.sr_post(info["path"] if info["fallback"] is None else f"{info['path']}:{info['fallback']}")
 
     def result(self) -> dict:
         return {"job_id": self.spec.job_id, "energy": self.energy.cpu().numpy(),
@@ -253,23 +281,19 @@ class ServeReport:
 class Server:
     """Continuous-batching VMC server on the current GPU."""
 
-    def __init__(self, hbm_budget: Optional[int] = None, gram: str = "bf16x3", workers: int = 1,
-                 keep_results: bool = False, progress=None):
+    def __init__(self, hbm_budget: Optional[int] = None, gram: str = "bf16x3", keep_results: bool = False,
+                 progress=None):
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info()
             hbm_budget = int(0.7 * free)
         self.budget = int(hbm_budget)
         self.gram = gram
-        self.workers = int(workers)
         self.keep_results = keep_results
         self.progress = progress
         self._ws = None
-        self._pool = ThreadPoolExecutor(max_workers=self.workers) if self.workers > 1 else None
 
     def close(self) -> None:
-        if self._pool is not None:
-            self._pool.shutdown()
-            self._pool = None
+        self._ws = None
 
     def _group_ws(self, njobs: int) -> torch.Tensor:
         need = max(int(_lib.lib().qtx_rbm_group_ws_bytes(njobs)), 1)
@@ -285,7 +309,9 @@ class Server:
         _lib.call(name, _lib.stream(), arr, len(jobs), ws.data_ptr(), ws.numel())
 
     def round(self, active: List[JobRun]) -> None:
-        """One VMC step of every active job."""
+        """One VMC step of every active job: grouped sampling, E_loc and
+        Jacobian on the current stream, then every job's solve on its own
+        stream through one qtx_minsr_solve_many call."""
         nst = [j.sweep_steps() for j in active]
         self._grouped("qtx_rbm_group_metropolis", active, nst)
         for j, n in zip(active, nst):
@@ -295,18 +321,22 @@ class Server:
         ready = torch.cuda.Event()
         ready.record()
         main = torch.cuda.current_stream()
-
-        def sr(job: JobRun):
-            with torch.cuda.stream(job.stream):
-                job.stream.wait_event(ready)
-                job.sr_update()
-                job.done_event.record(job.stream)
-
-        if self._pool is None:
-            for j in active:
-                sr(j)
-        else:
-            list(self._pool.map(sr, active))
+        for j in active:
+            with torch.cuda.stream(j.stream):
+                j.stream.wait_event(ready)
+                j.sr_pre()
+        tasks = (_lib.QtxMinsrTask * len(active))(*[j.sr_task() for j in active])
+        rc = _lib.lib().qtx_minsr_solve_many(tasks, len(active))
+        if rc != 0:
+            raise _lib.QtxError(f"qtx_minsr_solve_many failed ({rc}): "
+                                f"{_lib.lib().qtx_last_error().decode(errors='replace')}")
+        names = {0: "chol_shift", 1: "partial_fractions", 2: "syevd"}
+        reasons = {1: "small_cutoff", 2: "nonfinite", 3: "nonpositive_lambda_max", 4: "cholesky_failed"}
+        for j, tk in zip(active, tasks):
+            path = names.get(tk.path, "none") + (f":{reasons[tk.fallback]}" if tk.fallback else "")
+            with torch.cuda.stream(j.stream):
+                j.sr_post(path)
+                j.done_event.record(j.stream)
         for j in active:
             main.wait_event(j.done_event)
 

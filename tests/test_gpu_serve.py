@@ -26,7 +26,7 @@ def test_grouped_equals_single_job_path():
     sp = serve.JobSpec(job_id=0, L=6, J2=0.4, alpha=2, ns=128, nsteps=2, seed=77)
     a = serve.JobRun(sp, gram="fp64")
     b = serve.JobRun(sp, gram="fp64")
-    srv = serve.Server(workers=1)
+    srv = serve.Server()
     for _ in range(2):
         a.sample_single()
         nst = [b.sweep_steps()]
@@ -64,19 +64,19 @@ def _check_same(r1, r2):
 
 def test_alone_equals_mix():
     specs = _specs()
-    mix = _run(specs, workers=3)
+    mix = _run(specs)
     assert mix.total_steps == sum(s.nsteps for s in specs)
     assert len(mix.latencies) == 3 and mix.max_active == 3
     for sp in specs:
-        alone = _run([serve.JobSpec(**{**sp.__dict__, "arrival": 0.0})], workers=1)
+        alone = _run([serve.JobSpec(**{**sp.__dict__, "arrival": 0.0})])
         _check_same(alone.results[sp.job_id], mix.results[sp.job_id])
 
 
 def test_late_arrival_joins_running_jobs():
     specs = _specs()
     specs[2].arrival = 0.3  # arrives while the first two are running
-    late = _run(specs, workers=2)
-    ref = _run([serve.JobSpec(**{**specs[2].__dict__, "arrival": 0.0})], workers=1)
+    late = _run(specs)
+    ref = _run([serve.JobSpec(**{**specs[2].__dict__, "arrival": 0.0})])
     _check_same(ref.results[2], late.results[2])
     s = late.summary()
     assert s["jobs"] == 3 and s["latency_p99_s"] >= s["latency_p50_s"] > 0
@@ -85,5 +85,49 @@ def test_late_arrival_joins_running_jobs():
 def test_budget_limits_concurrency():
     specs = _specs()
     need = max(serve.footprint_bytes(s, "bf16x3") for s in specs)
-    rep = _run(specs, workers=2, hbm_budget=need)  # room for one job at a time
+    rep = _run(specs, hbm_budget=need)  # room for one job at a time
     assert rep.max_active == 1 and len(rep.latencies) == 3
+
+
+def test_solve_many_equals_single_solves(rng):
+    """qtx_minsr_solve_many on uneven tasks (MinSR and SR branches, three
+    modes) gives each task's qtx_minsr_solve result."""
+    from quantax_amd import _lib
+    dev = torch.device("cuda")
+    shapes = [(300, 1000, _lib.QTX_SOLVE_EIGH_PINV), (128, 2000, _lib.QTX_SOLVE_CHOL_SHIFT),
+              (500, 200, _lib.QTX_SOLVE_EIGH_PINV), (200, 700, _lib.QTX_SOLVE_EIGH_PINV_DIRECT),
+              (1024, 3000, _lib.QTX_SOLVE_EIGH_PINV), (900, 2500, _lib.QTX_SOLVE_CHOL_SHIFT),
+              (1500, 4000, _lib.QTX_SOLVE_EIGH_PINV)]
+    jobs = []
+    for ns, np_, mode in shapes:
+        O = torch.from_numpy((rng.standard_normal((ns, np_)) * 0.3 + 0.1).astype(np.float32)).to(dev)
+        E = torch.from_numpy(rng.standard_normal(ns) - 2.0).to(dev)
+        mean = torch.empty(np_, dtype=torch.float64, device=dev)
+        _lib.call("qtx_colmean", _lib.stream(), _lib.QTX_F32, O.data_ptr(), ns, np_, mean.data_ptr())
+        eb = torch.empty(ns, dtype=torch.float64, device=dev)
+        o2 = torch.empty(2, dtype=torch.float64, device=dev)
+        _lib.call("qtx_ebar", _lib.stream(), E.data_ptr(), None, ns, eb.data_ptr(), o2.data_ptr())
+        gram = _lib.QTX_GRAM_BF16X3
+        nws = int(_lib.lib().qtx_minsr_ws_bytes(ns, np_, mode, gram))
+        ws = torch.empty(nws, dtype=torch.uint8, device=dev)
+        step1 = torch.empty(np_, dtype=torch.float64, device=dev)
+        _lib.call("qtx_minsr_solve", _lib.stream(), _lib.QTX_F32, O.data_ptr(), mean.data_ptr(), None, eb.data_ptr(),
+                  ns, np_, mode, gram, 0.0, 1e-3, 1e-3, 0.0, step1.data_ptr(), ws.data_ptr(), nws)
+        jobs.append(dict(O=O, mean=mean, eb=eb, ws=ws, ns=ns, np=np_, mode=mode, gram=gram, ref=step1,
+                         out=torch.empty(np_, dtype=torch.float64, device=dev), stream=torch.cuda.Stream()))
+    torch.cuda.synchronize()
+    tasks = (_lib.QtxMinsrTask * len(jobs))()
+    for t, j in zip(tasks, jobs):
+        t.stream = j["stream"].cuda_stream
+        t.o_dtype, t.mode, t.gram = _lib.QTX_F32, j["mode"], j["gram"]
+        t.O, t.mean, t.r, t.ebar = j["O"].data_ptr(), j["mean"].data_ptr(), None, j["eb"].data_ptr()
+        t.ns, t.np = j["ns"], j["np"]
+        t.rshift, t.ashift, t.rtol, t.atol = 0.0, 1e-3, 1e-3, 0.0
+        t.step, t.ws, t.ws_bytes = j["out"].data_ptr(), j["ws"].data_ptr(), j["ws"].numel()
+    assert _lib.lib().qtx_minsr_solve_many(tasks, len(jobs)) == 0
+    torch.cuda.synchronize()
+    assert [t.path for t in tasks] == [1, 0, 1, 2, 1, 0, 1]
+    assert all(t.status == 0 and t.fallback == 0 for t in tasks)
+    for j in jobs:
+        err = (torch.linalg.norm(j["out"] - j["ref"]) / torch.linalg.norm(j["ref"])).item()
+        assert err < 1e-12, err
