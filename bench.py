@@ -46,6 +46,12 @@ def parse():
                    help="also attribute the mixed update-vector error to E_loc and to the Gram")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--no-serve", action="store_true", help="skip the served job stream")
+    p.add_argument("--serve-jobs", type=int, default=64)
+    p.add_argument("--serve-steps", type=int, default=10, help="MinSR steps per served job (the stream's jobs have 50)")
+    p.add_argument("--serve-gap", type=float, default=0.25, help="mean Poisson inter-arrival time (s)")
+    p.add_argument("--serve-seq-steps", type=int, default=3,
+                   help="steps per job timed for the sequential baseline (the rest extrapolated)")
     p.add_argument("--calib-copy-mb", type=int, default=0,
                    help="run a device-to-device copy of this many MiB first (HBM counter calibration)")
     return p.parse_args()
@@ -211,6 +217,33 @@ def precision_report(cfg, state, H, solver, last, torch, qtx, breakdown=False):
     return rep
 
 
+def run_serving(args, torch):
+    """The served job stream (64 J1-J2 RBM jobs, Poisson arrivals) against the
+    same jobs run one after another through the single-job path with the FP64
+    Gram; per-job cost is stationary, so the sequential total is extrapolated
+    from a few timed steps per job."""
+    from quantax_amd import serve
+
+    specs = serve.c5_stream(args.serve_jobs, mean_interarrival=args.serve_gap, nsteps=args.serve_steps, seed=5)
+    log = lambda m: print(f"[bench] serve: {m}", file=sys.stderr, flush=True)
+    srv = serve.Server(progress=log)
+    rep = srv.run(specs)
+    srv.close()
+    log(f"served: {rep.summary()}")
+    seq = serve.run_sequential(specs, gram="fp64", nsteps=args.serve_seq_steps, progress=log)
+    log(f"sequential: {seq.summary()}")
+    s, q = rep.summary(), seq.summary()
+    return {
+        "workload": (f"{args.serve_jobs} J1-J2 RBM_Dense jobs (L in 6..16, alpha in 1,2,4, ns in 1024..8192, "
+                     f"J2 in [0, 0.6]), Poisson arrivals, mean gap {args.serve_gap} s, {args.serve_steps} MinSR "
+                     f"steps each (split-bf16 Gram, soft pseudo-inverse rtol 1e-4, lr 1e-2)"),
+        "served": s,
+        "sequential_plain": {**q, "note": f"single-job path, FP64 Gram, {args.serve_seq_steps} steps per job timed, "
+                                          "rest extrapolated at the job's later-step cost"},
+        "served_over_sequential": round(s["steps_per_s"] / q["steps_per_s"], 3),
+    }
+
+
 def cpu_baseline(cfg, seconds):
     """Time the float64 oracle on a bounded sample of the workload on the host
     cores and convert it to VMC steps/s (forwards dominate: sweep + E_loc)."""
@@ -315,6 +348,8 @@ def main():
         out["precision"] = r["precision"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
+    if not args.no_serve and args.config == "c3":
+        out["serving"] = run_serving(args, torch)
     print(json.dumps(out))
 
 
