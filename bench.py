@@ -68,6 +68,16 @@ CONFIGS = {
 }
 
 
+# HBM traffic of one rc_metropolis_kernel launch at the 16x16 workload (mixed,
+# 512 steps, 16384 chains), from two rocprofv3 --pmc runs of this script
+# (FETCH_SIZE, then WRITE_SIZE, kernel-include-regex 'rc_metropolis|copyBuffer',
+# --steps 1 --warmup 0 --mode mixed --no-cpu-baseline --no-serve
+# --thermal-sweeps 0 --calib-copy-mb 256), each scaled by what the counter
+# reads for the 256 MiB device copy: FETCH 131092.7 KiB (scale 262144 /
+# 131092.7), WRITE 262144.0 KiB (scale 1).
+PMC_C3_MIXED = {"fetch_kib": 134363.3, "write_kib": 4352.0, "fetch_scale": 262144.0 / 131092.7, "write_scale": 1.0}
+
+
 def conv_flops_per_forward(nblocks, C, N):
     """Algorithmic FLOPs of one ResConv forward (multiply-add = 2)."""
     f = 2 * C * 1 * 9 * N  # first conv (Cin = 1)
@@ -311,8 +321,13 @@ def main():
         t = r["metropolis_kernel_ms"] / 1e3  # HIP events around the kernel launch, same stream
         ach = flops / t / 1e12
         peak = MFMA_PEAK["bf16"] if main_mode == "mixed" else MFMA_PEAK["fp32"]
+        traffic = None
+        if args.config == "c3" and main_mode == "mixed" and sweep_steps == 512:
+            p = PMC_C3_MIXED
+            traffic = int(1024 * (p["fetch_kib"] * p["fetch_scale"] + p["write_kib"] * p["write_scale"]))
         roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(ach / peak, 4), "traffic": None,
+                "frac": round(ach / peak, 4), "traffic": traffic,
+                "traffic_algorithmic": cfg["ns"] * sweep_steps * 16 + 2 * cfg["ns"] * N + 24 * cfg["ns"],
                 "kernel": "rc_metropolis_kernel", "flops_per_launch": flops,
                 "avg_launch_ms": round(r["metropolis_kernel_ms"], 3)}
     out = {

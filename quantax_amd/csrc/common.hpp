@@ -54,6 +54,17 @@ __device__ __forceinline__ double logcosh(double x) {
   return ax + log1p(exp(-2.0 * ax)) - 0.69314718055994530942;
 }
 
+// ---------------------------------------------------------------- XCD-aware block order
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// Remapping b to x * per + min(x, rem) + b / 8 (x = b % 8) gives each XCD a
+// contiguous range of virtual blocks, so data that neighbouring virtual
+// blocks share (cache lines of per-chain streams, row panels) stays in one
+// XCD's L2.  A bijection of [0, nblocks).
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
+  const int64_t per = nblocks / 8, rem = nblocks % 8, x = b % 8, loc = b / 8;
+  return x * per + (x < rem ? x : rem) + loc;
+}
+
 // ---------------------------------------------------------------- Philox4x32-10
 struct u32x4 {
   uint32_t x, y, z, w;

@@ -173,7 +173,7 @@ __global__ __launch_bounds__(256) void rbm_metropolis_kernel(
     int32_t* __restrict__ n_near) {
   __shared__ int8_t sbuf[WAVES_PER_BLOCK][MAXN];
   int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
-  int64_t c = (int64_t)blockIdx.x * WAVES_PER_BLOCK + wv;
+  int64_t c = xcd_block(blockIdx.x, gridDim.x) * WAVES_PER_BLOCK + wv;
   if (c >= ns) return;
   auto getw = [&](int t) { return reinterpret_cast<const uint4*>(words)[(int64_t)t * ns + c]; };
   rbm_chain_metropolis<T, R>(packed, M, N, kind, nbr, width, hopping, nsteps, getw, reweight, tie_tol, s + c * N,

@@ -996,7 +996,7 @@ __global__ __launch_bounds__(W * 64, ((W == 8 || ROWS == 2) ? 2 : 1)) void rc_lo
   __shared__ float red[2 * B * W];
   __shared__ double lp[B];
   const int N = a.g.N;
-  int64_t c0 = (int64_t)blockIdx.x * B;
+  const int64_t c0 = xcd_block(blockIdx.x, gridDim.x) * B;  // per-chain Philox words of a step share lines
   for (int i = threadIdx.x; i < B * N; i += (W * WAVE)) {
     int b = i / N;
     sp[i] = (c0 + b < ns) ? s[(c0 + b) * N + (i % N)] : (int8_t)1;
@@ -1022,7 +1022,7 @@ __global__ __launch_bounds__(W * 64, ((W == 8 || ROWS == 2) ? 2 : 1)) void rc_me
   __shared__ double mv_thr[B];
   const int N = a.g.N;
   const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
-  int64_t c0 = (int64_t)blockIdx.x * B;
+  const int64_t c0 = xcd_block(blockIdx.x, gridDim.x) * B;  // per-chain Philox words of a step share lines
   for (int i = threadIdx.x; i < B * N; i += (W * WAVE)) {
     int b = i / N;
     cur[i] = (c0 + b < ns) ? s[(c0 + b) * N + (i % N)] : (int8_t)1;

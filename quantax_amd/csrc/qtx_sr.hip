@@ -533,6 +533,41 @@ __global__ __launch_bounds__(256) void apply_update_kernel(T* __restrict__ theta
 // quadratic factored once by Cholesky; the residual uses T itself, so the
 // result has the accuracy of the well-conditioned complex system.
 
+// Y = T X for NV vectors (n x NV, column-major) with T symmetric n x n stored
+// in full: one wave per row, coalesced 8-byte reads of the row, X from cache.
+// An HBM stream of T at close to copy bandwidth (library GEMV / GEMM with
+// one or two columns reach a fraction of it).
+template <int NV>
+__global__ __launch_bounds__(256) void matvec_rows_kernel(const double* __restrict__ T, int64_t n,
+                                                          const double* __restrict__ X, double* __restrict__ Y) {
+  const int lane = threadIdx.x % WAVE;
+  const int64_t i = (int64_t)blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE;
+  if (i >= n) return;
+  const double* row = T + i * n;
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  int64_t j = lane;
+  for (; j + 3 * WAVE < n; j += 4 * WAVE) {
+    const double t0 = row[j], t1 = row[j + WAVE], t2 = row[j + 2 * WAVE], t3 = row[j + 3 * WAVE];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double* x = X + v * n + j;
+      acc[v] += t0 * x[0] + t1 * x[WAVE] + t2 * x[2 * WAVE] + t3 * x[3 * WAVE];
+    }
+  }
+  for (; j < n; j += WAVE) {
+    const double tj = row[j];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] += tj * X[v * n + j];
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const double s = wave_sum(acc[v]);
+    if (lane == 0) Y[v * n + i] = s;
+  }
+}
+
 // sum over the block (every thread gets the total; fixed summation order)
 __device__ __forceinline__ double block_sum(double x, double* red) {
   int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE, nw = blockDim.x / WAVE;
@@ -902,17 +937,15 @@ static int task_fail(Task& t, int rc) {
 }
 
 static int lanczos_enqueue(Task& t, rocblas_handle h, int to) {
-  const rocblas_int ni = (rocblas_int)t.n;
-  const double one = 1.0, zero = 0.0;
+  (void)h;
   double* alpha = t.w.ab;
   double* beta = t.w.ab + kLanczosMax;
+  const unsigned gm = (unsigned)((t.n + 3) / 4);
   for (int j = t.done; j < to; ++j) {
     double* v = t.w.V + (j % 3) * t.n;
     double* vp = t.w.V + ((j + 2) % 3) * t.n;
     double* vn = t.w.V + ((j + 1) % 3) * t.n;
-    int rc = rb(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &one, t.w.T, ni, v, 1, &zero, t.w.w, 1),
-                "rocblas_dgemv");
-    if (rc) return rc;
+    hipLaunchKernelGGL(matvec_rows_kernel<1>, dim3(gm), dim3(256), 0, t.st, t.w.T, t.n, v, t.w.w);
     hipLaunchKernelGGL(lanczos_step_kernel, dim3(1), dim3(1024), 0, t.st, t.n, j, t.w.w, v, vp, vn, alpha, beta);
   }
   QTX_LAUNCH_CHECK();
@@ -998,7 +1031,7 @@ static int phase2(Task& t, rocblas_handle h) {
     }
     t.lanczos = msteps;
     t.lmax = lmax;
-    const double c = io.rtol * lmax + io.atol, rel = c / lmax;
+    const double c = io.rtol * lmax + io.atol, rel = io.rtol + io.atol / lmax;  // c / lmax, exact for atol = 0
     if (bad) {
       t.fallback = 2;
     } else if (!(lmax > 0.0) || !std::isfinite(lmax)) {
@@ -1014,6 +1047,7 @@ static int phase2(Task& t, rocblas_handle h) {
       const rocblas_int ni = (rocblas_int)t.n;
       const double one = 1.0, zero = 0.0;
       const unsigned gb = (unsigned)nblk(t.n, 256);
+      const unsigned gm = (unsigned)((t.n + 3) / 4);
       int rc = rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, ni, ni, &one, t.w.T, ni, &zero,
                                 t.w.T2, ni),
                   "rocblas_dsyrk");
@@ -1031,16 +1065,10 @@ static int phase2(Task& t, rocblas_handle h) {
         for (int it = 0; it < iters; ++it) {
           rc = rb(rocsolver_dpotrs(h, rocblas_fill_lower, ni, 2, t.w.A, ni, t.w.R, ni), "rocsolver_dpotrs");
           if (rc) return rc;
-          rc = rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, ni, 2, ni, &one, t.w.T, ni, t.w.R,
-                                ni, &zero, t.w.TW, ni),
-                  "rocblas_dgemm");
-          if (rc) return rc;
+          hipLaunchKernelGGL(matvec_rows_kernel<2>, dim3(gm), dim3(256), 0, t.st, t.w.T, t.n, t.w.R, t.w.TW);
           hipLaunchKernelGGL(refine_update_kernel, dim3(gb), dim3(256), 0, t.st, t.w.R, t.w.TW, t.n, a, b, t.w.Z);
           if (it + 1 < iters) {
-            rc = rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, ni, 2, ni, &one, t.w.T, ni,
-                                  t.w.Z, ni, &zero, t.w.TW, ni),
-                    "rocblas_dgemm");
-            if (rc) return rc;
+            hipLaunchKernelGGL(matvec_rows_kernel<2>, dim3(gm), dim3(256), 0, t.st, t.w.T, t.n, t.w.Z, t.w.TW);
             hipLaunchKernelGGL(refine_residual_kernel, dim3(gb), dim3(256), 0, t.st, t.w.y, t.w.Z, t.w.TW, t.n, a, b,
                                t.w.R);
           }
