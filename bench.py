@@ -50,6 +50,8 @@ def parse():
     p.add_argument("--serve-jobs", type=int, default=64)
     p.add_argument("--serve-steps", type=int, default=10, help="MinSR steps per served job (the stream's jobs have 50)")
     p.add_argument("--serve-gap", type=float, default=0.25, help="mean Poisson inter-arrival time (s)")
+    p.add_argument("--serve-max-active", type=int, default=12,
+                   help="cap on concurrently served jobs (0: no cap); the cap lowers median latency")
     p.add_argument("--serve-seq-steps", type=int, default=3,
                    help="steps per job timed for the sequential baseline (the rest extrapolated)")
     p.add_argument("--calib-copy-mb", type=int, default=0,
@@ -236,7 +238,7 @@ def run_serving(args, torch):
 
     specs = serve.c5_stream(args.serve_jobs, mean_interarrival=args.serve_gap, nsteps=args.serve_steps, seed=5)
     log = lambda m: print(f"[bench] serve: {m}", file=sys.stderr, flush=True)
-    srv = serve.Server(progress=log)
+    srv = serve.Server(progress=log, max_active=args.serve_max_active or None)
     rep = srv.run(specs)
     srv.close()
     log(f"served: {rep.summary()}")
@@ -246,7 +248,8 @@ def run_serving(args, torch):
     return {
         "workload": (f"{args.serve_jobs} J1-J2 RBM_Dense jobs (L in 6..16, alpha in 1,2,4, ns in 1024..8192, "
                      f"J2 in [0, 0.6]), Poisson arrivals, mean gap {args.serve_gap} s, {args.serve_steps} MinSR "
-                     f"steps each (split-bf16 Gram, soft pseudo-inverse rtol 1e-4, lr 1e-2)"),
+                     f"steps each (split-bf16 Gram, soft pseudo-inverse rtol 1e-4, lr 1e-2), at most "
+                     f"{args.serve_max_active or 'unlimited'} jobs in flight"),
         "served": s,
         "sequential_plain": {**q, "note": f"single-job path, FP64 Gram, {args.serve_seq_steps} steps per job timed, "
                                           "rest extrapolated at the job's later-step cost"},

@@ -282,12 +282,13 @@ class Server:
     """Continuous-batching VMC server on the current GPU."""
 
     def __init__(self, hbm_budget: Optional[int] = None, gram: str = "bf16x3", keep_results: bool = False,
-                 progress=None):
+                 progress=None, max_active: Optional[int] = None):
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info()
             hbm_budget = int(0.7 * free)
         self.budget = int(hbm_budget)
         self.gram = gram
+        self.max_active = max_active  # optional cap on concurrent jobs (lower median latency)
         self.keep_results = keep_results
         self.progress = progress
         self._ws = None
@@ -353,7 +354,9 @@ class Server:
         t0 = time.perf_counter()
         while pending or active:
             now = time.perf_counter() - t0
-            for sp in admit(pending, now, used, self.budget, lambda s: footprint_bytes(s, self.gram)):
+            room = len(pending) if self.max_active is None else max(0, self.max_active - len(active))
+            for sp in admit(pending[:room], now, used, self.budget, lambda s: footprint_bytes(s, self.gram)):
+                pending.remove(sp)
                 used += footprint_bytes(sp, self.gram)
                 active.append(JobRun(sp, self.gram))
             if not active:
