@@ -736,6 +736,11 @@ using namespace qtx;
 static int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
 constexpr int kLanczosMax = 512;  // Lanczos steps for lambda_max before giving up on convergence
+#ifndef QTX_CHOL_BLOCK
+#define QTX_CHOL_BLOCK 1024
+#endif
+constexpr int64_t kCholBlock = QTX_CHOL_BLOCK;  // diagonal block of the blocked Cholesky
+constexpr int kInfoPerFactor = (int)((131072 + kCholBlock - 1) / kCholBlock);  // status slots: n <= 131072
 
 struct SolveWs {
   double* T;       // n x n
@@ -774,7 +779,7 @@ static size_t ws_layout(int64_t ns, int64_t np, int mode, int gram, SolveWs* w, 
   s.D = (double*)take((size_t)n * 8);
   s.E = (double*)take((size_t)n * 8);
   s.scalars = (double*)take(64);
-  s.info = (int32_t*)take(64);
+  s.info = (int32_t*)take(3 * kInfoPerFactor * sizeof(int32_t));
   if (mode == QTX_SOLVE_EIGH_PINV) {
     s.T2 = (double*)take((size_t)n * n * 8);
     s.A = (double*)take((size_t)n * n * 8);
@@ -896,6 +901,36 @@ static int pinv_syevd(hipStream_t st, rocblas_handle h, int64_t n, double rtol, 
   return 1;
 }
 
+// A = L L^T (lower, column-major, leading dimension n) by diagonal blocks of
+// kCholBlock: rocSOLVER potrf on the block, rocBLAS dtrsm for the panel below
+// it and dsyrk for the trailing matrix, so nearly all flops run in level-3
+// BLAS kernels (a whole-matrix potrf spends most of its time in small panel
+// factorisations).  info[b] = potrf status of diagonal block b (0: factored).
+static int blocked_potrf(rocblas_handle h, int64_t n, double* A, int32_t* info) {
+  const double one = 1.0, minus_one = -1.0;
+  const rocblas_int ld = (rocblas_int)n;
+  int b = 0;
+  for (int64_t k = 0; k < n; k += kCholBlock, ++b) {
+    const int64_t kb = (n - k < kCholBlock) ? n - k : kCholBlock;
+    double* Akk = A + k * n + k;
+    int rc = rb(rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)kb, Akk, ld, info + b), "rocsolver_dpotrf");
+    if (rc) return rc;
+    const int64_t m = n - k - kb;
+    if (m == 0) break;
+    double* Apan = Akk + kb;
+    rc = rb(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                          rocblas_diagonal_non_unit, (rocblas_int)m, (rocblas_int)kb, &one, Akk, ld, Apan, ld),
+            "rocblas_dtrsm");
+    if (rc) return rc;
+    double* A22 = A + (k + kb) * n + (k + kb);
+    rc = rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, (rocblas_int)m, (rocblas_int)kb, &minus_one,
+                          Apan, ld, &one, A22, ld),
+            "rocblas_dsyrk");
+    if (rc) return rc;
+  }
+  return QTX_OK;
+}
+
 // ---------------------------------------------------------------- phased solve
 // A solve runs as three phases so that many solves on different streams can
 // be driven from one host thread with batched host syncs
@@ -975,7 +1010,9 @@ static int phase1(Task& t, rocblas_handle h) {
     hipLaunchKernelGGL(shift_diag_kernel, dim3(1), dim3(1024), 0, t.st, t.w.T, t.n, io.rshift, io.ashift,
                        t.w.scalars);
     QTX_LAUNCH_CHECK();
-    rc = rb(rocsolver_dpotrf(h, rocblas_fill_lower, ni, t.w.T, ni, t.w.info), "rocsolver_dpotrf");
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(nblk(3 * kInfoPerFactor, 256)), dim3(256), 0, t.st, t.w.info,
+                       (int64_t)(3 * kInfoPerFactor), 0);
+    rc = blocked_potrf(h, t.n, t.w.T, t.w.info);
     if (rc) return rc;
     return rb(rocsolver_dpotrs(h, rocblas_fill_lower, ni, 1, t.w.T, ni, t.w.y, ni), "rocsolver_dpotrs");
   }
@@ -1053,14 +1090,15 @@ static int phase2(Task& t, rocblas_handle h) {
                                 t.w.T2, ni),
                   "rocblas_dsyrk");
       if (rc) return rc;
-      hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, t.st, t.w.info, (int64_t)4, 0);
+      hipLaunchKernelGGL(fill_i32_kernel, dim3(nblk(3 * kInfoPerFactor, 256)), dim3(256), 0, t.st, t.w.info,
+                         (int64_t)(3 * kInfoPerFactor), 0);
       for (int k = 0; k < 3; ++k) {
         const double th = 3.14159265358979323846 * (2 * k + 1) / 6.0;
         const double a = (k == 1) ? 0.0 : c * cos(th), b = c * sin(th);
         hipLaunchKernelGGL(form_quad_kernel, dim3((unsigned)((t.n * t.n + 255) / 256)), dim3(256), 0, t.st, t.w.T2,
                            t.w.T, t.n, 2.0 * a, c * c, t.w.A);
         QTX_LAUNCH_CHECK();
-        rc = rb(rocsolver_dpotrf(h, rocblas_fill_lower, ni, t.w.A, ni, t.w.info + k), "rocsolver_dpotrf");
+        rc = blocked_potrf(h, t.n, t.w.A, t.w.info + k * kInfoPerFactor);
         if (rc) return rc;
         hipLaunchKernelGGL(refine_init_kernel, dim3(gb), dim3(256), 0, t.st, t.w.y, t.n, t.w.R, t.w.Z);
         for (int it = 0; it < iters; ++it) {
@@ -1092,13 +1130,19 @@ template <typename TI>
 static int phase3(Task& t, rocblas_handle h) {
   qtx_minsr_task& io = *t.io;
   if (io.mode == QTX_SOLVE_CHOL_SHIFT || t.pf) {
-    int32_t info_h[4] = {0, 0, 0, 0};
-    QTX_HIP(hipMemcpyAsync(info_h, t.w.info, sizeof(info_h), hipMemcpyDeviceToHost, t.st));
+    std::vector<int32_t> info_h(3 * kInfoPerFactor);
+    QTX_HIP(hipMemcpyAsync(info_h.data(), t.w.info, info_h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, t.st));
     QTX_HIP(hipStreamSynchronize(t.st));
+    int bad = -1;  // first diagonal block whose factorisation failed
+    const int nslots = (io.mode == QTX_SOLVE_CHOL_SHIFT) ? kInfoPerFactor : 3 * kInfoPerFactor;
+    for (int i = 0; i < nslots && bad < 0; ++i)
+      if (info_h[i] != 0) bad = i;
     if (io.mode == QTX_SOLVE_CHOL_SHIFT) {
-      if (info_h[0] != 0) return fail(QTX_ESOLVER, "potrf info = " + std::to_string(info_h[0]));
+      if (bad >= 0)
+        return fail(QTX_ESOLVER, "potrf failed in diagonal block " + std::to_string(bad) +
+                                     " (info " + std::to_string(info_h[bad]) + ")");
       t.path = 0;
-    } else if (info_h[0] || info_h[1] || info_h[2]) {  // T and y are intact: take the dsyevd route
+    } else if (bad >= 0) {  // T and y are intact: take the dsyevd route
       t.fallback = 4;
       int rc = pinv_syevd(t.st, h, t.n, io.rtol, io.atol, t.w);
       if (rc < 0) return rc;
@@ -1127,7 +1171,7 @@ static int check_task(const qtx_minsr_task& io) {
               "unknown solver mode");
   QTX_REQUIRE(io.gram == QTX_GRAM_FP64 || io.gram == QTX_GRAM_FP32 || io.gram == QTX_GRAM_BF16X3,
               "unknown gram mode");
-  QTX_REQUIRE((io.ns < io.np ? io.ns : io.np) <= 0x7fffffff, "matrix too large for rocSOLVER");
+  QTX_REQUIRE((io.ns < io.np ? io.ns : io.np) <= 131072, "matrix too large (n > 131072)");
   size_t need = ws_layout(io.ns, io.np, io.mode, io.gram, nullptr, nullptr);
   if (io.ws == nullptr || io.ws_bytes < need)
     return fail(QTX_EWORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
