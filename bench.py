@@ -12,7 +12,7 @@ accumulation), FP64 centring/eigh/update; "plain" = FP32 MFMA convolutions and
 FP64 MFMA Gram.  The JSON line reports the mixed mode as ``value`` and the
 plain mode beside it.
 
-    python bench.py [--steps K] [--warmup W] [--mode mixed|plain|both] [--config c3|c2|c1]
+    python bench.py [--steps K] [--warmup W] [--mode mixed|plain|both] [--config c3|c2|c1|c4]
 """
 from __future__ import annotations
 
@@ -34,9 +34,10 @@ def parse():
     p.add_argument("--steps", type=int, default=2)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--mode", default="both", choices=["mixed", "plain", "both"])
-    p.add_argument("--config", default="c3", choices=["c3", "c2", "c1"])
-    p.add_argument("--sweep-steps", type=int, default=None, help="proposals per chain per sweep (default 2N)")
-    p.add_argument("--thermal-sweeps", type=int, default=1)
+    p.add_argument("--config", default="c3", choices=["c3", "c2", "c1", "c4"])
+    p.add_argument("--sweep-steps", type=int, default=None,
+                   help="proposals per chain per sweep (default 2N; 8 for c4)")
+    p.add_argument("--thermal-sweeps", type=int, default=None, help="thermalisation sweeps (default 1; 0 for c4)")
     p.add_argument("--solver", default="eigh", choices=["eigh", "eigh-direct", "chol"],
                    help="eigh: soft pseudo-inverse (partial fractions when the cutoff allows); "
                         "eigh-direct: the same through dsyevd; chol: diagonal-shift Cholesky")
@@ -67,6 +68,13 @@ CONFIGS = {
                workload="Heisenberg 10x10 (MSR, Sz=0), RBM_Dense(alpha=4), 4096 chains, MinSR"),
     "c1": dict(L=16, model="rbm1d", features=16, ns=1024,
                workload="TFIM chain L=16 (h=1), RBM_Dense(16) float64, 1024 chains, SR"),
+    # the large case: sampling is cut to 8 proposals per chain and no
+    # thermalisation (a full 2N sweep is 6 PFLOP of forwards); E_loc, O, the
+    # 32768 x 32768 Gram and the solve run in full
+    "c4": dict(L=20, model="resconv", nblocks=16, channels=32, ns=32768, J=[1.0, 0.5], nn=[1, 2], msr=True,
+               sweep_steps=8, thermal_sweeps=0, nconn_est=800,
+               workload="J1-J2 20x20 (J2=0.5, MSR, Sz=0), ResConv(16 blocks, 32 ch), 32768 chains, MinSR "
+                        "(T 32768^2); sampling cut to 8 proposals per chain, no thermalisation"),
 }
 
 
@@ -184,8 +192,12 @@ def run_mode(cfg, mode, args, torch, qtx):
                metropolis_kernel_ms=float(np.mean(kms)) if kms else None,
                energies=energies, acceptance=sampler.acceptance, nconn=float(H._connectivity.float().mean().item()),
                nparams=state.nparams, solve_info=last.get("solve_info"))
+    out["eloc_ms"] = out["stages"]["eloc"]
     if mode == "mixed" and cfg["model"] == "resconv":
-        out["precision"] = precision_report(cfg, state, H, solver, last, torch, qtx, args.precision_breakdown)
+        if cfg["L"] == 16:
+            out["precision"] = precision_report(cfg, state, H, solver, last, torch, qtx, args.precision_breakdown)
+        else:
+            out["gram"] = gram_report(last, solver, torch, qtx, opt._ws)
     return out
 
 
@@ -229,6 +241,48 @@ def precision_report(cfg, state, H, solver, last, torch, qtx, breakdown=False):
     return rep
 
 
+def gram_report(last, solver, torch, qtx, ws_solve):
+    """The large case: MFMA rate of the split-bf16 Gram entry point on the last
+    step's O, and the update-vector error of the mixed solve against the same
+    route with the FP64 Gram, both on the last step's O and E_loc (the errors of
+    the network precision are measured on the 16x16 workload)."""
+    from quantax_amd import _lib
+
+    O, mean, eb = last["O"], last["mean"], last["ebar"]
+    ns, np_ = O.shape
+    g = _lib.QTX_GRAM_BF16X3
+    nws = int(_lib.lib().qtx_gram_ws_bytes(ns, np_, g))
+    T = torch.empty((ns, ns), dtype=torch.float64, device=O.device)
+    ws = torch.empty(nws, dtype=torch.uint8, device=O.device)
+    ev = []
+    for _ in range(2):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        _lib.call("qtx_gram", _lib.stream(), _lib.QTX_F32, O.data_ptr(), mean.data_ptr(), None, ns, np_, g,
+                  T.data_ptr(), ws.data_ptr(), nws)
+        b.record()
+        ev.append((a, b))
+    torch.cuda.synchronize()
+    ms = min(a.elapsed_time(b) for a, b in ev)
+    del T, ws
+    torch.cuda.empty_cache()
+    useful = float(ns) * ns * np_  # lower triangle (the kernel computes 128 x 128 tiles on and below the diagonal)
+    rep = {"kernel": "qtx_gram (split-bf16: centring/split pass + MFMA Gram)", "ms": round(ms, 2),
+           "algorithmic_flops": useful, "achieved_tflops": round(useful / ms / 1e9, 1),
+           "bf16_mfma_issue_tflops": round(3 * useful / ms / 1e9, 1),
+           "bf16_mfma_utilisation": round(3 * useful / ms / 1e9 / MFMA_PEAK["bf16"], 4),
+           "flop_convention": "Ns^2 Np (SYRK-halved, one multiply-add = 2 flops over the lower triangle); "
+                              "issue = 3 split products"}
+    print(f"[bench] gram: {ms:.1f} ms, {rep['achieved_tflops']} TFLOP/s algorithmic", file=sys.stderr, flush=True)
+    ref_solver = qtx.optimizer.Solver(solver.mode, solver.rshift, solver.ashift, solver.rtol, solver.atol, "fp64")
+    x_m = qtx.optimizer.solve_step(O, mean, eb, solver, None, ws_solve)
+    x_p = qtx.optimizer.solve_step(O, mean, eb, ref_solver)
+    torch.cuda.empty_cache()
+    rep["update_vector_rel_err"] = (torch.linalg.norm(x_m - x_p) / torch.linalg.norm(x_p)).item()
+    rep["reference"] = "same O and E_loc, FP64 MFMA Gram, same soft-inverse route"
+    return rep
+
+
 def run_serving(args, torch):
     """The served job stream (64 J1-J2 RBM jobs, Poisson arrivals) against the
     same jobs run one after another through the single-job path with the FP64
@@ -257,7 +311,7 @@ def run_serving(args, torch):
     }
 
 
-def cpu_baseline(cfg, seconds):
+def cpu_baseline(cfg, seconds, sweep_steps):
     """Time the float64 oracle on a bounded sample of the workload on the host
     cores and convert it to VMC steps/s (forwards dominate: sweep + E_loc)."""
     import oracle.models as om
@@ -280,7 +334,8 @@ def cpu_baseline(cfg, seconds):
     dt = time.perf_counter() - t0
     fwd_per_s = n / dt
     ns = cfg["ns"]
-    fwd_per_step = ns * (2 * N) + ns * 550  # sweep proposals + ~550 connected configurations per sample
+    # sweep proposals + connected configurations per sample (~550 at 16x16, ~800 at 20x20)
+    fwd_per_step = ns * sweep_steps + ns * cfg.get("nconn_est", 550)
     return dict(value=fwd_per_s / fwd_per_step, unit="VMC steps/s (extrapolated)", cores=cores, kind="port",
                 sample=f"{n} ResConv forwards of the float64 numpy oracle in {dt:.1f} s on {cores} cores "
                        f"({fwd_per_s:.1f} forwards/s), scaled by {fwd_per_step} forwards per VMC step")
@@ -306,6 +361,10 @@ def main():
         print(f"[bench] calibration copy of {args.calib_copy_mb} MiB done", file=sys.stderr, flush=True)
         del src, dst
     cfg = CONFIGS[args.config]
+    if args.sweep_steps is None:
+        args.sweep_steps = cfg.get("sweep_steps")
+    if args.thermal_sweeps is None:
+        args.thermal_sweeps = cfg.get("thermal_sweeps", 1)
     modes = ["mixed", "plain"] if args.mode == "both" else [args.mode]
     res = {}
     for m in modes:
@@ -318,7 +377,17 @@ def main():
     N = cfg["L"] ** 2 if cfg["model"] != "rbm1d" else cfg["L"]
     sweep_steps = args.sweep_steps or 2 * N
     roof = None
-    if cfg["model"] == "resconv":
+    if cfg["model"] == "resconv" and args.config == "c4":
+        # sampling is cut here, so the dominant kernel is the fused E_loc (one
+        # forward per valid connected configuration); HIP events on its stream
+        fwd = conv_flops_per_forward(cfg["nblocks"], cfg["channels"], N)
+        flops = cfg["ns"] * r["nconn"] * fwd
+        ach = flops / (r["eloc_ms"] / 1e3) / 1e12
+        peak = MFMA_PEAK["bf16"] if main_mode == "mixed" else MFMA_PEAK["fp32"]
+        roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(ach / peak, 4), "traffic": None, "kernel": "rc_eloc_kernel",
+                "flops_per_launch": flops, "avg_launch_ms": round(r["eloc_ms"], 3)}
+    elif cfg["model"] == "resconv":
         fwd = conv_flops_per_forward(cfg["nblocks"], cfg["channels"], N)
         flops = cfg["ns"] * sweep_steps * fwd  # one Metropolis launch = one sweep
         t = r["metropolis_kernel_ms"] / 1e3  # HIP events around the kernel launch, same stream
@@ -364,8 +433,10 @@ def main():
         out["mixed_over_plain"] = round(r["steps_per_s"] / p["steps_per_s"], 3)
     if "precision" in r:
         out["precision"] = r["precision"]
+    if "gram" in r:
+        out["gram"] = r["gram"]
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds, sweep_steps)
     if not args.no_serve and args.config == "c3":
         out["serving"] = run_serving(args, torch)
     print(json.dumps(out))

@@ -251,20 +251,24 @@ static inline int64_t g3_rows(int64_t n) { return (n + GT - 1) / GT * GT; }
 static inline int64_t g3_kpad(int64_t np) { return (np + G3_KC - 1) / G3_KC * G3_KC; }
 static inline size_t g3_bytes(int64_t n, int64_t np) { return (size_t)g3_rows(n) * g3_kpad(np) * 2 * sizeof(__bf16); }
 
+// Grid-stride over (row, 256-column block) pairs: a launch's work-item count
+// is a 32-bit quantity, and rows * kpad passes 2^32 at 32768 x 286976.
 template <typename TI>
 __global__ __launch_bounds__(256) void split_bf16x3_kernel(const TI* __restrict__ O, const double* __restrict__ mean,
                                                            const double* __restrict__ r, int64_t n, int64_t np,
-                                                           int64_t kpad, int64_t bpr, double inv_ns,
+                                                           int64_t kpad, int64_t bpr, int64_t nblocks, double inv_ns,
                                                            __bf16* __restrict__ Hi, __bf16* __restrict__ Lo) {
-  const int64_t row = blockIdx.x / bpr;
-  const int64_t k = (blockIdx.x % bpr) * 256 + threadIdx.x;
-  if (k >= kpad) return;
-  double v = 0.0;
-  if (row < n && k < np) v = ((double)O[row * np + k] - mean[k]) * sqrt((r ? r[row] : 1.0) * inv_ns);
-  const float f = (float)v;
-  const __bf16 hi = (__bf16)f;
-  Hi[row * kpad + k] = hi;
-  Lo[row * kpad + k] = (__bf16)(f - (float)hi);
+  for (int64_t blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
+    const int64_t row = blk / bpr;
+    const int64_t k = (blk % bpr) * 256 + threadIdx.x;
+    if (k >= kpad) continue;
+    double v = 0.0;
+    if (row < n && k < np) v = ((double)O[row * np + k] - mean[k]) * sqrt((r ? r[row] : 1.0) * inv_ns);
+    const float f = (float)v;
+    const __bf16 hi = (__bf16)f;
+    Hi[row * kpad + k] = hi;
+    Lo[row * kpad + k] = (__bf16)(f - (float)hi);
+  }
 }
 
 // lower-triangle tile t of the XCD-contiguous order: workgroups are dealt to
@@ -469,8 +473,9 @@ __global__ __launch_bounds__(1024) void pow2_scale_kernel(const double* __restri
 }
 
 __global__ void scale_by_kernel(double* __restrict__ T, int64_t n2, const double* __restrict__ scale) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n2) T[i] *= scale[0];
+  const double sc = scale[0];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x)
+    T[i] *= sc;
 }
 
 // soft inverse of eigenvalues times rho (solver.py:94-101), single block
@@ -638,10 +643,10 @@ __global__ __launch_bounds__(1024) void lanczos_step_kernel(int64_t n, int j, do
 __global__ __launch_bounds__(256) void form_quad_kernel(const double* __restrict__ T2,
                                                         const double* __restrict__ T, int64_t n, double two_a,
                                                         double c2, double* __restrict__ A) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n * n) return;
-  int64_t j = e / n, i = e - j * n;
-  A[e] = (i < j) ? 0.0 : T2[e] - two_a * T[e] + (i == j ? c2 : 0.0);
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t j = e / n, i = e - j * n;
+    A[e] = (i < j) ? 0.0 : T2[e] - two_a * T[e] + (i == j ? c2 : 0.0);
+  }
 }
 
 // R = [b, 0], Z = 0   (R, Z: n x 2 column-major)
@@ -734,6 +739,9 @@ static int rb(rocblas_status s, const char* what) {
 using namespace qtx;
 
 static int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
+// grid cap of the grid-stride element-wise kernels (256-thread blocks): the
+// work-item count of one launch must stay below 2^32
+constexpr int64_t kGridCap = int64_t(1) << 20;
 
 constexpr int kLanczosMax = 512;  // Lanczos steps for lambda_max before giving up on convergence
 #ifndef QTX_CHOL_BLOCK
@@ -842,11 +850,11 @@ static int launch_gram_nt(hipStream_t st, int gram, const TI* O, const double* m
   if (gram == QTX_GRAM_BF16X3) {
     QTX_REQUIRE(split_ws != nullptr, "split-bf16 Gram needs its workspace");
     const int64_t rows = g3_rows(n), kpad = g3_kpad(np), bpr = (kpad + 255) / 256;
-    QTX_REQUIRE(rows * bpr < 0x7fffffff, "Gram too large");
+    const int64_t nblocks = rows * bpr;
     __bf16* Hi = (__bf16*)split_ws;
     __bf16* Lo = Hi + rows * kpad;
-    hipLaunchKernelGGL(split_bf16x3_kernel<TI>, dim3((unsigned)(rows * bpr)), dim3(256), 0, st, O, mean, r, n, np,
-                       kpad, bpr, inv_ns, Hi, Lo);
+    hipLaunchKernelGGL(split_bf16x3_kernel<TI>, dim3((unsigned)std::min<int64_t>(nblocks, kGridCap)), dim3(256), 0,
+                       st, O, mean, r, n, np, kpad, bpr, nblocks, inv_ns, Hi, Lo);
     hipLaunchKernelGGL(gram_bf16x3_kernel, g, bl, 0, st, (const __bf16*)Hi, (const __bf16*)Lo, n, kpad, (int)ntiles,
                        T);
   } else if (gram == QTX_GRAM_FP32) {
@@ -878,7 +886,8 @@ static int pinv_syevd(hipStream_t st, rocblas_handle h, int64_t n, double rtol, 
   // exact power-of-two rescaling of T (eigenvectors unchanged, eigenvalues
   // scaled by 2^k and undone in eig_inv_kernel)
   hipLaunchKernelGGL(pow2_scale_kernel, dim3(1), dim3(1024), 0, st, w.T, n, w.scalars + 1);
-  hipLaunchKernelGGL(scale_by_kernel, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, st, w.T, n * n,
+  hipLaunchKernelGGL(scale_by_kernel, dim3((unsigned)std::min<int64_t>((n * n + 255) / 256, kGridCap)), dim3(256), 0,
+                     st, w.T, n * n,
                      w.scalars + 1);
   QTX_LAUNCH_CHECK();
   int rc = rb(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, ni, w.T, ni, w.D, w.E, w.info),
@@ -1095,7 +1104,8 @@ static int phase2(Task& t, rocblas_handle h) {
       for (int k = 0; k < 3; ++k) {
         const double th = 3.14159265358979323846 * (2 * k + 1) / 6.0;
         const double a = (k == 1) ? 0.0 : c * cos(th), b = c * sin(th);
-        hipLaunchKernelGGL(form_quad_kernel, dim3((unsigned)((t.n * t.n + 255) / 256)), dim3(256), 0, t.st, t.w.T2,
+        hipLaunchKernelGGL(form_quad_kernel, dim3((unsigned)std::min<int64_t>((t.n * t.n + 255) / 256, kGridCap)),
+                           dim3(256), 0, t.st, t.w.T2,
                            t.w.T, t.n, 2.0 * a, c * c, t.w.A);
         QTX_LAUNCH_CHECK();
         rc = blocked_potrf(h, t.n, t.w.A, t.w.info + k * kInfoPerFactor);

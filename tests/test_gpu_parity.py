@@ -458,3 +458,29 @@ def test_gram_entry_point(gram, tol, shape, rng):
     Tn = T.cpu().numpy()
     assert np.array_equal(Tn, Tn.T)
     assert np.max(np.abs(Tn - ref)) / np.max(np.abs(ref)) < tol
+
+
+def test_gram_split_beyond_2pow32_elements():
+    """ns * np above 2^32 (the split pass once launched rows * kpad work-items,
+    which wraps a launch's 32-bit grid size).  O = a (x) u with a_s in
+    {-2..2} is exact in FP32, so T_st = (a_s - abar)(a_t - abar) sum(u^2) / ns
+    in closed form; the workspace starts as NaN, so any element the split pass
+    skips poisons T."""
+    ns, np_ = 256, (1 << 24) + 4096
+    assert ns * np_ > (1 << 32)
+    gen = torch.Generator(device=DEV).manual_seed(7)
+    a = torch.randint(-2, 3, (ns,), generator=gen, device=DEV).to(torch.float32)
+    u = torch.randn(np_, generator=gen, device=DEV, dtype=torch.float32)
+    O = a[:, None] * u[None, :]
+    mean = torch.empty(np_, dtype=torch.float64, device=DEV)
+    _lib.call("qtx_colmean", _lib.stream(), _lib.QTX_F32, O.data_ptr(), ns, np_, mean.data_ptr())
+    nws = int(_lib.lib().qtx_gram_ws_bytes(ns, np_, _lib.QTX_GRAM_BF16X3))
+    ws = torch.full((nws,), 0xFF, dtype=torch.uint8, device=DEV)
+    T = torch.empty((ns, ns), dtype=torch.float64, device=DEV)
+    _lib.call("qtx_gram", _lib.stream(), _lib.QTX_F32, O.data_ptr(), mean.data_ptr(), None, ns, np_,
+              _lib.QTX_GRAM_BF16X3, T.data_ptr(), ws.data_ptr(), nws)
+    del O, ws
+    ad = a.double() - a.double().mean()
+    ref = torch.outer(ad, ad) * (u.double() ** 2).sum() / ns
+    assert torch.isfinite(T).all()
+    assert ((T - ref).abs().max() / ref.abs().max()).item() < 1e-5
