@@ -640,9 +640,9 @@ __global__ __launch_bounds__(1024) void lanczos_step_kernel(int64_t n, int j, do
 }
 
 // A = T2 - 2a T + c^2 I on the lower triangle (column-major, i >= j), 0 above
-__global__ __launch_bounds__(256) void form_quad_kernel(const double* __restrict__ T2,
+__global__ __launch_bounds__(256) void form_quad_kernel(const double* T2,
                                                         const double* __restrict__ T, int64_t n, double two_a,
-                                                        double c2, double* __restrict__ A) {
+                                                        double c2, double* A) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += (int64_t)gridDim.x * blockDim.x) {
     int64_t j = e / n, i = e - j * n;
     A[e] = (i < j) ? 0.0 : T2[e] - two_a * T[e] + (i == j ? c2 : 0.0);
@@ -759,10 +759,10 @@ struct SolveWs {
   double* scalars; // 8
   int32_t* info;   // 4 (+pad)
   // partial-fraction soft inverse (EIGH_PINV modes only)
-  double* T2;      // n x n: T^2 (lower)
-  double* A;       // n x n: (T - a)^2 + b^2, then its Cholesky factor
-  double* R;       // n x 2: residual, then A^{-1} residual
-  double* TW;      // n x 2: T times a pair of vectors
+  double* T2;      // 3 x (n x n): T^2 (lower), then the three (T - a_k)^2 + b_k^2
+                   // (the first in place of T^2) and their Cholesky factors
+  double* R;       // n x 6: residuals, then A_k^{-1} residuals (pole k: columns 2k, 2k+1)
+  double* TW;      // n x 6: T times those vectors
   double* Z;       // n x 2: complex solution
   double* x;       // n: accumulated soft-inverse solution
   double* V;       // 3n: Lanczos vectors
@@ -789,11 +789,10 @@ static size_t ws_layout(int64_t ns, int64_t np, int mode, int gram, SolveWs* w, 
   s.scalars = (double*)take(64);
   s.info = (int32_t*)take(3 * kInfoPerFactor * sizeof(int32_t));
   if (mode == QTX_SOLVE_EIGH_PINV) {
-    s.T2 = (double*)take((size_t)n * n * 8);
-    s.A = (double*)take((size_t)n * n * 8);
-    s.R = (double*)take((size_t)n * 16);
-    s.TW = (double*)take((size_t)n * 16);
-    s.Z = (double*)take((size_t)n * 16);
+    s.T2 = (double*)take((size_t)3 * n * n * 8);
+    s.R = (double*)take((size_t)n * 48);
+    s.TW = (double*)take((size_t)n * 48);
+    s.Z = (double*)take((size_t)n * 48);
     s.x = (double*)take((size_t)n * 8);
     s.V = (double*)take((size_t)n * 24);
     s.w = (double*)take((size_t)n * 8);
@@ -910,32 +909,208 @@ static int pinv_syevd(hipStream_t st, rocblas_handle h, int64_t n, double rtol, 
   return 1;
 }
 
-// A = L L^T (lower, column-major, leading dimension n) by diagonal blocks of
-// kCholBlock: rocSOLVER potrf on the block, rocBLAS dtrsm for the panel below
-// it and dsyrk for the trailing matrix, so nearly all flops run in level-3
-// BLAS kernels (a whole-matrix potrf spends most of its time in small panel
-// factorisations).  info[b] = potrf status of diagonal block b (0: factored).
-static int blocked_potrf(rocblas_handle h, int64_t n, double* A, int32_t* info) {
+// ---------------------------------------------------------------- Cholesky
+// Right-looking blocked Cholesky (lower, column-major, leading dimension n) of
+// `batch` matrices `stride` elements apart.  Each kCholBlock-wide diagonal
+// block is factored in CT-column panels by two kernels of this file (panel:
+// factor the CT x CT diagonal tile in LDS and solve the tiles below it inside
+// the block; update: the rank-CT update of the block's remaining tiles), all
+// matrices in one launch, so the latency-bound diagonal work of one matrix
+// overlaps that of the others; the rows below a diagonal block are then solved
+// by rocBLAS dtrsm and the trailing matrix updated by dsyrk, matrix by matrix,
+// so nearly all flops run in level-3 BLAS kernels.  info[b * info_stride + blk]
+// receives 1 + the column (within diagonal block blk) of the first
+// non-positive pivot of matrix b, 0 if none; as in LAPACK, that matrix's
+// factor is not usable then (NaNs propagate, the other matrices are
+// unaffected).
+constexpr int CT = 64;
+
+// Thread (r, q) = (lane, wave) holds row r, columns 16q .. 16q+15 of a 64 x 64
+// tile in registers.  Column step j: the wave owning column j publishes it to
+// LDS (two alternating buffers, one barrier per step), every thread reads the
+// pivot and the column entries of its own rows/columns and updates its 16
+// values.  The column loop runs as 4 groups of 16 unrolled steps, so every
+// register index and every "column after j" test is static inside a group; a
+// wave's role in a group (its columns all after the group, the group's own
+// columns, or all before it) is wave-uniform.  Every workgroup factors the
+// panel's diagonal tile; workgroup 0 stores it, workgroup t >= 1 solves the
+// t-th tile below it inside the diagonal block, X = A_t L^{-T}, by forward
+// substitution along the columns in the same pattern.
+__global__ __launch_bounds__(256) void chol_panel_kernel(double* A, int64_t lda, int64_t stride, int64_t d0,
+                                                         int64_t kb, int kp, int32_t* info, int64_t info_stride) {
+  __shared__ double Ls[CT][CT + 1];
+  __shared__ double col[2][CT];
+  __shared__ double rdg[CT];
+  double* Ab = A + (int64_t)blockIdx.y * stride;
+  const int tid = threadIdx.x, r = tid % CT;
+  const int q = __builtin_amdgcn_readfirstlane(tid / CT), c0 = 16 * q;
+  const int64_t p0 = d0 + (int64_t)kp * CT;
+  const int w = (int)(kb - (int64_t)kp * CT < CT ? kb - (int64_t)kp * CT : CT);
+  double a[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int c = c0 + i;
+    double v = (r == c) ? 1.0 : 0.0;  // padding: identity
+    if (r < w && c < w && r >= c) v = Ab[(p0 + r) + (p0 + c) * lda];
+    a[i] = v;
+  }
+  int bad = -1;
+  for (int jb = 0; jb < 4; ++jb) {
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) {
+      const int j = 16 * jb + jj;
+      double* cb = col[jj & 1];
+      if (q == jb) cb[r] = a[jj];
+      __syncthreads();
+      const double djj = cb[j];
+      if (!(djj > 0.0) && bad < 0 && j < w) bad = j;  // NaN fails too
+      const double rs = 1.0 / sqrt(djj);
+      const double lr = cb[r] * (rs * rs);  // L_rj L_cj = A_rj A_cj / d_jj
+      if (q > jb) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) a[i] -= lr * cb[c0 + i];
+      } else if (q == jb) {
+#pragma unroll
+        for (int i = jj + 1; i < 16; ++i) a[i] -= lr * cb[c0 + i];
+        a[jj] *= rs;
+        if (r == 0) rdg[j] = rs;
+      }
+    }
+  }
+  if (blockIdx.x == 0) {
+    if (tid == 0 && bad >= 0) {
+      int32_t* slot = info + (int64_t)blockIdx.y * info_stride;
+      if (*slot == 0) *slot = kp * CT + bad + 1;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int c = c0 + i;
+      if (r < w && c < w && r >= c) Ab[(p0 + r) + (p0 + c) * lda] = a[i];
+    }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < 16; ++i) Ls[r][c0 + i] = a[i];
+  const int64_t q0 = p0 + (int64_t)blockIdx.x * CT;
+  const int64_t hrem = kb - (int64_t)(kp + blockIdx.x) * CT;
+  const int h = (int)(hrem < CT ? hrem : CT);
+  double x[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int c = c0 + i;
+    x[i] = (r < h && c < w) ? Ab[(q0 + r) + (p0 + c) * lda] : 0.0;
+  }
+  __syncthreads();  // Ls and rdg complete; the column buffers are free again
+  for (int jb = 0; jb < 4; ++jb) {
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) {
+      const int c = 16 * jb + jj;
+      double* cb = col[jj & 1];
+      if (q == jb) {
+        x[jj] *= rdg[c];
+        cb[r] = x[jj];
+      }
+      __syncthreads();
+      const double xc = cb[r];
+      if (q > jb) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] -= xc * Ls[c0 + i][c];
+      } else if (q == jb) {
+#pragma unroll
+        for (int i = jj + 1; i < 16; ++i) x[i] -= xc * Ls[c0 + i][c];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int c = c0 + i;
+    if (r < h && c < w) Ab[(q0 + r) + (p0 + c) * lda] = x[i];
+  }
+}
+
+// A_ij -= L_i L_j^T for the tiles (i, j), kp < j <= i, of the diagonal block
+// (lower triangle only on diagonal tiles); one 64 x 64 tile per workgroup,
+// 4 x 4 outputs per thread
+__global__ __launch_bounds__(256) void chol_update_kernel(double* A, int64_t lda, int64_t stride, int64_t d0,
+                                                          int64_t kb, int kp) {
+  __shared__ double Li[CT][CT + 1];
+  __shared__ double Lj[CT][CT + 1];
+  double* Ab = A + (int64_t)blockIdx.y * stride;
+  const int tid = threadIdx.x;
+  int ti, tj;
+  tile_of(blockIdx.x, ti, tj);
+  const int i = kp + 1 + ti, j = kp + 1 + tj;
+  const int64_t p0 = d0 + (int64_t)kp * CT;
+  const int w = (int)(kb - (int64_t)kp * CT < CT ? kb - (int64_t)kp * CT : CT);
+  const int64_t ri = d0 + (int64_t)i * CT, rj = d0 + (int64_t)j * CT;
+  const int hi = (int)(kb - (int64_t)i * CT < CT ? kb - (int64_t)i * CT : CT);
+  const int hj = (int)(kb - (int64_t)j * CT < CT ? kb - (int64_t)j * CT : CT);
+  for (int e = tid; e < CT * CT; e += 256) {
+    const int r = e % CT, l = e / CT;
+    Li[r][l] = (r < hi && l < w) ? Ab[(ri + r) + (p0 + l) * lda] : 0.0;
+    Lj[r][l] = (r < hj && l < w) ? Ab[(rj + r) + (p0 + l) * lda] : 0.0;
+  }
+  __syncthreads();
+  const int r0 = (tid % 16) * 4, c0 = (tid / 16) * 4;
+  double acc[4][4] = {};
+  for (int l = 0; l < w; ++l) {
+    double a[4], b[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      a[q] = Li[r0 + q][l];
+      b[q] = Lj[c0 + q][l];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[q][u] += a[q] * b[u];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int c = c0 + u;
+    if (c >= hj) continue;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = r0 + q;
+      if (r < hi && (i != j || r >= c)) Ab[(ri + r) + (rj + c) * lda] -= acc[q][u];
+    }
+  }
+}
+
+static int blocked_potrf(rocblas_handle h, hipStream_t st, int64_t n, double* A, int64_t stride, int batch,
+                         int32_t* info, int64_t info_stride) {
   const double one = 1.0, minus_one = -1.0;
   const rocblas_int ld = (rocblas_int)n;
   int b = 0;
   for (int64_t k = 0; k < n; k += kCholBlock, ++b) {
     const int64_t kb = (n - k < kCholBlock) ? n - k : kCholBlock;
-    double* Akk = A + k * n + k;
-    int rc = rb(rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)kb, Akk, ld, info + b), "rocsolver_dpotrf");
-    if (rc) return rc;
+    const int nt = (int)((kb + CT - 1) / CT);
+    for (int kp = 0; kp < nt; ++kp) {
+      hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)(nt - kp), (unsigned)batch), dim3(256), 0, st, A,
+                         (int64_t)n, stride, k, kb, kp, info + b, info_stride);
+      const int m = nt - kp - 1;
+      if (m > 0)
+        hipLaunchKernelGGL(chol_update_kernel, dim3((unsigned)(m * (m + 1) / 2), (unsigned)batch), dim3(256), 0, st,
+                           A, (int64_t)n, stride, k, kb, kp);
+    }
+    QTX_LAUNCH_CHECK();
     const int64_t m = n - k - kb;
     if (m == 0) break;
-    double* Apan = Akk + kb;
-    rc = rb(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
-                          rocblas_diagonal_non_unit, (rocblas_int)m, (rocblas_int)kb, &one, Akk, ld, Apan, ld),
-            "rocblas_dtrsm");
-    if (rc) return rc;
-    double* A22 = A + (k + kb) * n + (k + kb);
-    rc = rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, (rocblas_int)m, (rocblas_int)kb, &minus_one,
-                          Apan, ld, &one, A22, ld),
-            "rocblas_dsyrk");
-    if (rc) return rc;
+    // one call per matrix: the strided-batched forms of these two routines
+    // run on much slower non-GEMM kernels at these sizes
+    for (int bi = 0; bi < batch; ++bi) {
+      double* Akk = A + bi * stride + k * n + k;
+      double* Apan = Akk + kb;
+      int rc = rb(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                                rocblas_diagonal_non_unit, (rocblas_int)m, (rocblas_int)kb, &one, Akk, ld, Apan, ld),
+                  "rocblas_dtrsm");
+      if (rc) return rc;
+      double* A22 = A + bi * stride + (k + kb) * n + (k + kb);
+      rc = rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, (rocblas_int)m, (rocblas_int)kb,
+                            &minus_one, Apan, ld, &one, A22, ld),
+              "rocblas_dsyrk");
+      if (rc) return rc;
+    }
   }
   return QTX_OK;
 }
@@ -1021,7 +1196,7 @@ static int phase1(Task& t, rocblas_handle h) {
     QTX_LAUNCH_CHECK();
     hipLaunchKernelGGL(fill_i32_kernel, dim3(nblk(3 * kInfoPerFactor, 256)), dim3(256), 0, t.st, t.w.info,
                        (int64_t)(3 * kInfoPerFactor), 0);
-    rc = blocked_potrf(h, t.n, t.w.T, t.w.info);
+    rc = blocked_potrf(h, t.st, t.n, t.w.T, 0, 1, t.w.info, kInfoPerFactor);
     if (rc) return rc;
     return rb(rocsolver_dpotrs(h, rocblas_fill_lower, ni, 1, t.w.T, ni, t.w.y, ni), "rocsolver_dpotrs");
   }
@@ -1101,30 +1276,55 @@ static int phase2(Task& t, rocblas_handle h) {
       if (rc) return rc;
       hipLaunchKernelGGL(fill_i32_kernel, dim3(nblk(3 * kInfoPerFactor, 256)), dim3(256), 0, t.st, t.w.info,
                          (int64_t)(3 * kInfoPerFactor), 0);
+      // pole k = c e^{i pi (2k+1)/6}: quadratic k into matrix k of the T2 block,
+      // k = 0 last because it overwrites T^2 in place
+      const int64_t nn = t.n * t.n;
+      double pa[3], pb[3];
       for (int k = 0; k < 3; ++k) {
         const double th = 3.14159265358979323846 * (2 * k + 1) / 6.0;
-        const double a = (k == 1) ? 0.0 : c * cos(th), b = c * sin(th);
-        hipLaunchKernelGGL(form_quad_kernel, dim3((unsigned)std::min<int64_t>((t.n * t.n + 255) / 256, kGridCap)),
-                           dim3(256), 0, t.st, t.w.T2,
-                           t.w.T, t.n, 2.0 * a, c * c, t.w.A);
-        QTX_LAUNCH_CHECK();
-        rc = blocked_potrf(h, t.n, t.w.A, t.w.info + k * kInfoPerFactor);
-        if (rc) return rc;
-        hipLaunchKernelGGL(refine_init_kernel, dim3(gb), dim3(256), 0, t.st, t.w.y, t.n, t.w.R, t.w.Z);
-        for (int it = 0; it < iters; ++it) {
-          rc = rb(rocsolver_dpotrs(h, rocblas_fill_lower, ni, 2, t.w.A, ni, t.w.R, ni), "rocsolver_dpotrs");
-          if (rc) return rc;
-          hipLaunchKernelGGL(matvec_rows_kernel<2>, dim3(gm), dim3(256), 0, t.st, t.w.T, t.n, t.w.R, t.w.TW);
-          hipLaunchKernelGGL(refine_update_kernel, dim3(gb), dim3(256), 0, t.st, t.w.R, t.w.TW, t.n, a, b, t.w.Z);
-          if (it + 1 < iters) {
-            hipLaunchKernelGGL(matvec_rows_kernel<2>, dim3(gm), dim3(256), 0, t.st, t.w.T, t.n, t.w.Z, t.w.TW);
-            hipLaunchKernelGGL(refine_residual_kernel, dim3(gb), dim3(256), 0, t.st, t.w.y, t.w.Z, t.w.TW, t.n, a, b,
-                               t.w.R);
-          }
-        }
-        hipLaunchKernelGGL(refine_accum_kernel, dim3(gb), dim3(256), 0, t.st, t.w.Z, t.n, k == 0 ? 1 : 0, t.w.x);
+        pa[k] = (k == 1) ? 0.0 : c * cos(th);
+        pb[k] = c * sin(th);
+      }
+      const unsigned gq = (unsigned)std::min<int64_t>((nn + 255) / 256, kGridCap);
+      for (int k : {1, 2, 0}) {
+        hipLaunchKernelGGL(form_quad_kernel, dim3(gq), dim3(256), 0, t.st, t.w.T2, t.w.T, t.n, 2.0 * pa[k], c * c,
+                           t.w.T2 + k * nn);
         QTX_LAUNCH_CHECK();
       }
+      rc = blocked_potrf(h, t.st, t.n, t.w.T2, nn, 3, t.w.info, kInfoPerFactor);
+      if (rc) return rc;
+      // iterative refinement of the three complex systems (T - p_k) z = y,
+      // the inner solves batched over the three factors, the products with T
+      // over all six vectors in one pass
+      for (int k = 0; k < 3; ++k)
+        hipLaunchKernelGGL(refine_init_kernel, dim3(gb), dim3(256), 0, t.st, t.w.y, t.n, t.w.R + 2 * k * t.n,
+                           t.w.Z + 2 * k * t.n);
+      for (int it = 0; it < iters; ++it) {
+        rc = rb(rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                                              rocblas_diagonal_non_unit, ni, 2, &one, t.w.T2, ni, nn, t.w.R, ni,
+                                              2 * t.n, 3),
+                "rocblas_dtrsm_strided_batched");
+        if (rc) return rc;
+        rc = rb(rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
+                                              rocblas_diagonal_non_unit, ni, 2, &one, t.w.T2, ni, nn, t.w.R, ni,
+                                              2 * t.n, 3),
+                "rocblas_dtrsm_strided_batched");
+        if (rc) return rc;
+        hipLaunchKernelGGL(matvec_rows_kernel<6>, dim3(gm), dim3(256), 0, t.st, t.w.T, t.n, t.w.R, t.w.TW);
+        for (int k = 0; k < 3; ++k)
+          hipLaunchKernelGGL(refine_update_kernel, dim3(gb), dim3(256), 0, t.st, t.w.R + 2 * k * t.n,
+                             t.w.TW + 2 * k * t.n, t.n, pa[k], pb[k], t.w.Z + 2 * k * t.n);
+        if (it + 1 < iters) {
+          hipLaunchKernelGGL(matvec_rows_kernel<6>, dim3(gm), dim3(256), 0, t.st, t.w.T, t.n, t.w.Z, t.w.TW);
+          for (int k = 0; k < 3; ++k)
+            hipLaunchKernelGGL(refine_residual_kernel, dim3(gb), dim3(256), 0, t.st, t.w.y, t.w.Z + 2 * k * t.n,
+                               t.w.TW + 2 * k * t.n, t.n, pa[k], pb[k], t.w.R + 2 * k * t.n);
+        }
+      }
+      for (int k = 0; k < 3; ++k)
+        hipLaunchKernelGGL(refine_accum_kernel, dim3(gb), dim3(256), 0, t.st, t.w.Z + 2 * k * t.n, t.n, k == 0 ? 1 : 0,
+                           t.w.x);
+      QTX_LAUNCH_CHECK();
       t.pf = true;
     }
   }

@@ -341,6 +341,44 @@ def test_pinv_partial_fractions(rtol, shape, rng):
     assert np.linalg.norm(step2 - ref) / np.linalg.norm(ref) < 1e-10
 
 
+@pytest.mark.parametrize("shape,rtol", [((2500, 4000), 1e-4), ((1100, 1500), 1e-2)])
+def test_pinv_partial_fractions_several_blocks(shape, rtol, rng):
+    """ns above the 1024-wide diagonal blocks of the batched Cholesky (the
+    last block partial): panel solves, strided-batched trsm and syrk."""
+    ns, np_ = shape
+    O, E = _solver_case(ns, np_, np.float32, rng)
+    O[:, : np_ // 2] *= 0.05
+    Od, mean, eb = _solve_inputs(O, E)
+    Obar, _ = osr.get_Obar(O.astype(np.float64))
+    Ebar, _, _ = osr.get_Ebar(E)
+    ref = osr.auto_pinv_eig(Obar, Ebar, rtol=rtol)
+    step = qtx.optimizer.solve_step(Od, mean, eb, qtx.optimizer.auto_pinv_eig(rtol=rtol, gram="fp64")).cpu().numpy()
+    assert qtx.optimizer.last_solve_info()["path"] == "partial_fractions"
+    assert np.linalg.norm(step - ref) / np.linalg.norm(ref) < 1e-10
+
+
+def test_chol_shift_several_blocks(rng):
+    ns, np_ = 2100, 3000
+    O, E = _solver_case(ns, np_, np.float32, rng)
+    Od, mean, eb = _solve_inputs(O, E)
+    Obar, _ = osr.get_Obar(O.astype(np.float64))
+    Ebar, _, _ = osr.get_Ebar(E)
+    ref = osr.auto_shift_eig(Obar, Ebar, rshift=0.0, ashift=1e-3)
+    step = qtx.optimizer.solve_step(Od, mean, eb, qtx.optimizer.auto_shift_eig(rshift=0.0, ashift=1e-3)).cpu().numpy()
+    assert qtx.optimizer.last_solve_info()["path"] == "chol_shift"
+    assert np.linalg.norm(step - ref) / np.linalg.norm(ref) < 1e-10
+
+
+def test_chol_shift_indefinite_reports_block(rng):
+    """A negative shift larger than the smallest eigenvalue makes T + shift
+    indefinite: the factorisation reports the diagonal block of the first
+    non-positive pivot and the solve fails loudly."""
+    O, E = _solver_case(1500, 3000, np.float32, rng)
+    Od, mean, eb = _solve_inputs(O, E)
+    with pytest.raises(_lib.QtxError, match="potrf failed in diagonal block"):
+        qtx.optimizer.solve_step(Od, mean, eb, qtx.optimizer.auto_shift_eig(rshift=0.0, ashift=-1e3))
+
+
 def test_pinv_small_cutoff_uses_syevd(rng):
     O, E = _solver_case(128, 500, np.float32, rng)
     Od, mean, eb = _solve_inputs(O, E)
