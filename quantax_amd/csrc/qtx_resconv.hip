@@ -1167,66 +1167,134 @@ __global__ __launch_bounds__(W * 64, ((W == 8 || ROWS == 2) ? 2 : 1)) void rc_el
 
 
 // ---------------------------------------------------------------- Jacobian
-// One workgroup per sample.  Forward on VALU records, per block i, x_i (block
-// input), y_i (conv1 input), z1_i (conv1 output) and a1_i (conv2 input) in
-// channel-major [c][p] float layout; the backward then produces the O row
-// (ravel_pytree order: per layer weight (co, ci, tap), bias (co)).
+// One workgroup (4 waves) per sample.  Every convolution of the forward
+// recompute and of the backward pass is an FP32-MFMA GEMM (16x16x4 f32) whose
+// operands are gathered from channel-major [c][p] activations in LDS through a
+// table of the 9 periodic neighbours of every site:
+//   forward   Z[co][p]     = sum_{ci,tap} W[co][ci][tap] X[ci][nb(p, tap)]
+//   input grad dX[ci][q]   = sum_{co,tap} W[co][ci][tap] dZ[co][nb(q, 8 - tap)]
+//   weight grad G[co][ci,tap] = sum_p dZ[co][p] X[ci][nb(p, tap)]
+// (MFMA lane l holds A[m0 + l%16][k0 + l/16], B[k0 + l/16][n0 + l%16]; result
+// element i of lane l is D[m0 + 4(l/16) + i][n0 + l%16].)  The forward records,
+// per block i, x_i (block input), y_i (conv1 input), z1_i (conv1 output) and
+// a1_i (conv2 input) in HBM workspace; the backward produces the O row in
+// ravel_pytree order (per layer weight (co, ci, tap), bias (co)).
 constexpr int JAC_THREADS = 256;
 
-template <int C>
-__device__ __forceinline__ void jac_conv_fwd(const float* __restrict__ in, int cin, const float* __restrict__ W,
-                                             const float* __restrict__ bias, const RcGeom& g, int p, float* out) {
-  int y = p / g.L1, x = p % g.L1;
-#pragma unroll
-  for (int co = 0; co < C; ++co) out[co] = bias ? bias[co] : 0.0f;
-  for (int ci = 0; ci < cin; ++ci)
-#pragma unroll
-    for (int tap = 0; tap < 9; ++tap) {
-      float v = in[ci * g.N + tap_pos(g, y, x, tap)];
-#pragma unroll
-      for (int co = 0; co < C; ++co) out[co] += W[(co * cin + ci) * 9 + tap] * v;
-    }
+__device__ __forceinline__ v4f mfma_f32_4(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// d in[ci][q] = sum_{co, tap} W[co][ci][tap] dz[co][q - shift(tap)]
-template <int C>
-__device__ __forceinline__ void jac_conv_bwd_in(const float* __restrict__ dz, int cin, const float* __restrict__ W,
-                                                const RcGeom& g, int q, float* din) {
-  int y = q / g.L1, x = q % g.L1;
-  for (int ci = 0; ci < cin; ++ci) din[ci] = 0.0f;
+// out[co][p] for all co < C, p < N; K = cin * 9; A fragments (weights) of a
+// 16-row tile held in registers across the wave's column tiles.
+template <int C, int KS, typename Epi>
+__device__ __forceinline__ void jac_conv_fwd_mfma(const float* __restrict__ in, const float* __restrict__ W,
+                                                  const int16_t* __restrict__ nbr, int N, int nmax, Epi epi) {
+  constexpr int K = (KS == 3) ? 9 : C * 9;  // cin * 9
+  const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+  const int ntile = (N + 15) / 16;
 #pragma unroll
-  for (int tap = 0; tap < 9; ++tap) {
-    int ps = tap_pos(g, y, x, 8 - tap);  // position p with shift_tap(p) = q
-    for (int co = 0; co < C; ++co) {
-      float d = dz[co * g.N + ps];
-      for (int ci = 0; ci < cin; ++ci) din[ci] += W[(co * cin + ci) * 9 + tap] * d;
+  for (int m0 = 0; m0 < C; m0 += 16) {
+    float af[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int k = 4 * ks + lane / 16;
+      af[ks] = (k < K) ? W[(m0 + lane % 16) * K + k] : 0.0f;
+    }
+    for (int t = wv; t < ntile; t += JAC_THREADS / WAVE) {
+      const int p = 16 * t + lane % 16;
+      const int pc = p < N ? p : N - 1;
+      v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const int k = 4 * ks + lane / 16;
+        const int ci = k / 9, tap = k - 9 * ci;
+        const float b = (k < K) ? in[ci * N + nbr[tap * nmax + pc]] : 0.0f;
+        acc = mfma_f32_4(af[ks], b, acc);
+      }
+      const int pw = 16 * t + lane % 16;
+      if (pw < N) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) epi(m0 + 4 * (lane / 16) + i, pw, acc[i]);
+      }
     }
   }
 }
 
-// grads of W (co, ci, tap) and bias for one layer, written to the O row
-template <int C>
-__device__ __forceinline__ void jac_conv_grad_w(const float* __restrict__ dz, const float* __restrict__ in, int cin,
-                                                const RcGeom& g, float* __restrict__ Orow, int64_t w0, int64_t b0) {
-  for (int pair = threadIdx.x; pair < C * cin; pair += JAC_THREADS) {
-    int co = pair / cin, ci = pair % cin;
-    float acc[9];
+// din[ci][q] for all ci < C (input channels of a C -> C convolution)
+template <int C, typename Epi>
+__device__ __forceinline__ void jac_conv_bwd_in_mfma(const float* __restrict__ dz, const float* __restrict__ W,
+                                                     const int16_t* __restrict__ nbr, int N, int nmax, Epi epi) {
+  constexpr int KS = C * 9 / 4;
+  const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+  const int ntile = (N + 15) / 16;
 #pragma unroll
-    for (int tap = 0; tap < 9; ++tap) acc[tap] = 0.0f;
-    for (int p = 0; p < g.N; ++p) {
-      float d = dz[co * g.N + p];
-      int y = p / g.L1, x = p % g.L1;
+  for (int m0 = 0; m0 < C; m0 += 16) {
+    float af[KS];
+    const int ci = m0 + lane % 16;
 #pragma unroll
-      for (int tap = 0; tap < 9; ++tap) acc[tap] += d * in[ci * g.N + tap_pos(g, y, x, tap)];
+    for (int ks = 0; ks < KS; ++ks) {
+      const int k = 4 * ks + lane / 16;
+      const int co = k / 9, tap = k - 9 * co;
+      af[ks] = W[(co * C + ci) * 9 + tap];
     }
+    for (int t = wv; t < ntile; t += JAC_THREADS / WAVE) {
+      const int q = 16 * t + lane % 16;
+      const int qc = q < N ? q : N - 1;
+      v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int tap = 0; tap < 9; ++tap) Orow[w0 + (int64_t)pair * 9 + tap] = acc[tap];
+      for (int ks = 0; ks < KS; ++ks) {
+        const int k = 4 * ks + lane / 16;
+        const int co = k / 9, tap = k - 9 * co;
+        acc = mfma_f32_4(af[ks], dz[co * N + nbr[(8 - tap) * nmax + qc]], acc);
+      }
+      if (q < N) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) epi(m0 + 4 * (lane / 16) + i, q, acc[i]);
+      }
+    }
+  }
+}
+
+// G[co][n], n = ci * 9 + tap < cin * 9, written to the O row; bias grads by VALU
+template <int C>
+__device__ __forceinline__ void jac_conv_grad_w_mfma(const float* __restrict__ dz, const float* __restrict__ x, int cin,
+                                                     const int16_t* __restrict__ nbr, int N, int nmax,
+                                                     float* __restrict__ Orow, int64_t w0, int64_t b0) {
+  const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+  const int K9 = cin * 9;
+  const int ncol = (K9 + 15) / 16;
+  const int ntiles = (C / 16) * ncol;
+  const int ksteps = (N + 3) / 4;
+  for (int t = wv; t < ntiles; t += JAC_THREADS / WAVE) {
+    const int m0 = 16 * (t % (C / 16)), n0 = 16 * (t / (C / 16));
+    const int co = m0 + lane % 16;
+    const int n = n0 + lane % 16;
+    const bool nok = n < K9;
+    const int ci = nok ? n / 9 : 0, tap = nok ? n - 9 * (n / 9) : 0;
+    const float* xr = x + ci * N;
+    const int16_t* nr = nbr + tap * nmax;
+    const float* dr = dz + co * N;
+    v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 4
+    for (int ks = 0; ks < ksteps; ++ks) {
+      const int p = 4 * ks + lane / 16;
+      const bool pok = p < N;
+      const float a = pok ? dr[p] : 0.0f;
+      const float b = (pok && nok) ? xr[nr[pok ? p : 0]] : 0.0f;
+      acc = mfma_f32_4(a, b, acc);
+    }
+    if (nok) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) Orow[w0 + (int64_t)(m0 + 4 * (lane / 16) + i) * K9 + n] = acc[i];
+    }
   }
   if (b0 >= 0) {
-    for (int co = threadIdx.x; co < C; co += JAC_THREADS) {
+    for (int c = wv; c < C; c += JAC_THREADS / WAVE) {
       float acc = 0.0f;
-      for (int p = 0; p < g.N; ++p) acc += dz[co * g.N + p];
-      Orow[b0 + co] = acc;
+      for (int p = lane; p < N; p += WAVE) acc += dz[c * N + p];
+      acc = wave_sum(acc);
+      if (lane == 0) Orow[b0 + c] = acc;
     }
   }
 }
@@ -1237,9 +1305,10 @@ __global__ __launch_bounds__(JAC_THREADS) void rc_jacobian_kernel(RcGeom g, int 
                                                                   const int8_t* __restrict__ s, int64_t s0,
                                                                   int64_t ns_chunk, int64_t np,
                                                                   float* __restrict__ O, float* __restrict__ ws) {
-  __shared__ float bufX[C * NMAX];
-  __shared__ float bufA[C * NMAX];
-  __shared__ float bufB[C * NMAX];
+  __shared__ float buf0[C * NMAX];
+  __shared__ float buf1[C * NMAX];
+  __shared__ float buf2[C * NMAX];
+  __shared__ int16_t nbr[9 * NMAX];
   __shared__ float red[2 * JAC_THREADS / WAVE];
   const int64_t nl = blockIdx.x;
   if (nl >= ns_chunk) return;
@@ -1249,9 +1318,16 @@ __global__ __launch_bounds__(JAC_THREADS) void rc_jacobian_kernel(RcGeom g, int 
   float* rec = ws + nl * blk_f * nb;
   float* Orow = O + n * np;
   const float isq2 = 0.70710678118654752f;
+  float* bX = buf0;  // block input / running gradient
+  float* bA = buf1;
+  float* bB = buf2;
 
+  for (int e = threadIdx.x; e < 9 * N; e += JAC_THREADS) {
+    const int tap = e / N, p = e - tap * N;
+    nbr[tap * NMAX + p] = (int16_t)tap_pos(g, p / g.L1, p % g.L1, tap);
+  }
   // ---- forward with record
-  for (int p = threadIdx.x; p < N; p += JAC_THREADS) bufX[p] = (float)s[n * N + p];
+  for (int p = threadIdx.x; p < N; p += JAC_THREADS) bX[p] = (float)s[n * N + p];
   __syncthreads();
   for (int i = 0; i < nb; ++i) {
     const int cin = (i == 0) ? 1 : C;
@@ -1261,53 +1337,59 @@ __global__ __launch_bounds__(JAC_THREADS) void rc_jacobian_kernel(RcGeom g, int 
     float* ra = rz + C * N;
     const float sc = rsqrtf((float)(i + 1));
     for (int e = threadIdx.x; e < cin * N; e += JAC_THREADS) {
-      float xv = bufX[e];
+      float xv = bX[e];
       rx[e] = xv;
       float xin = xv * sc;
       float yv = (i == 0) ? xin * isq2 : gelu_f(xin, gelu_exact);
-      bufA[e] = yv;
+      bA[e] = yv;
       ry[e] = yv;
     }
     __syncthreads();
     int64_t w1, b1, w2, b2;
     rc_theta_offsets(nb, C, 2 * i, w1, b1);
     rc_theta_offsets(nb, C, 2 * i + 1, w2, b2);
-    float out[C];
-    for (int p = threadIdx.x; p < N; p += JAC_THREADS) {
-      jac_conv_fwd<C>(bufA, cin, theta + w1, theta + b1, g, p, out);
-#pragma unroll
-      for (int co = 0; co < C; ++co) {
-        rz[co * N + p] = out[co];
-        bufB[co * N + p] = gelu_f(out[co], gelu_exact);
-        ra[co * N + p] = bufB[co * N + p];
-      }
+    {
+      const float* bias = theta + b1;
+      auto epi = [&](int co, int p, float v) {
+        v += bias[co];
+        rz[co * N + p] = v;
+        const float a = gelu_f(v, gelu_exact);
+        bB[co * N + p] = a;
+        ra[co * N + p] = a;
+      };
+      if (i == 0)
+        jac_conv_fwd_mfma<C, 3>(bA, theta + w1, nbr, N, NMAX, epi);
+      else
+        jac_conv_fwd_mfma<C, C * 9 / 4>(bA, theta + w1, nbr, N, NMAX, epi);
     }
     __syncthreads();
-    for (int p = threadIdx.x; p < N; p += JAC_THREADS) {
-      jac_conv_fwd<C>(bufB, C, theta + w2, b2 >= 0 ? theta + b2 : nullptr, g, p, out);
-      float r0 = bufX[p];  // channel 0 of the residual (block 0: the spin)
-#pragma unroll
-      for (int co = 0; co < C; ++co) out[co] += (cin == 1) ? r0 : bufX[co * N + p];
-#pragma unroll
-      for (int co = 0; co < C; ++co) bufA[co * N + p] = out[co];
+    {
+      const float* bias = (b2 >= 0) ? theta + b2 : nullptr;
+      auto epi = [&](int co, int p, float v) {
+        if (bias) v += bias[co];
+        v += (cin == 1) ? bX[p] : bX[co * N + p];  // residual (block 0: the spin, broadcast)
+        bA[co * N + p] = v;
+      };
+      jac_conv_fwd_mfma<C, C * 9 / 4>(bB, theta + w2, nbr, N, NMAX, epi);
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < C * N; e += JAC_THREADS) bufX[e] = bufA[e];
-    __syncthreads();
+    float* t = bX;
+    bX = bA;
+    bA = t;
   }
   // ---- head: dX = softmax(h) / sqrt(nb + 1), h = x / sqrt(nb + 1)
   const float hs = rsqrtf((float)(nb + 1));
   {
     const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
     float m = -INFINITY;
-    for (int e = threadIdx.x; e < C * N; e += JAC_THREADS) m = fmaxf(m, bufX[e] * hs);
+    for (int e = threadIdx.x; e < C * N; e += JAC_THREADS) m = fmaxf(m, bX[e] * hs);
     m = wave_max(m);
     if (lane == 0) red[wv] = m;
     __syncthreads();
     m = -INFINITY;
     for (int w = 0; w < JAC_THREADS / WAVE; ++w) m = fmaxf(m, red[w]);
     float ssum = 0.0f;
-    for (int e = threadIdx.x; e < C * N; e += JAC_THREADS) ssum += expf(bufX[e] * hs - m);
+    for (int e = threadIdx.x; e < C * N; e += JAC_THREADS) ssum += expf(bX[e] * hs - m);
     ssum = wave_sum(ssum);
     __syncthreads();
     if (lane == 0) red[wv] = ssum;
@@ -1315,7 +1397,7 @@ __global__ __launch_bounds__(JAC_THREADS) void rc_jacobian_kernel(RcGeom g, int 
     ssum = 0.0f;
     for (int w = 0; w < JAC_THREADS / WAVE; ++w) ssum += red[w];
     float inv = 1.0f / ssum;
-    for (int e = threadIdx.x; e < C * N; e += JAC_THREADS) bufX[e] = expf(bufX[e] * hs - m) * inv * hs;
+    for (int e = threadIdx.x; e < C * N; e += JAC_THREADS) bX[e] = expf(bX[e] * hs - m) * inv * hs;
     __syncthreads();
   }
   // ---- backward
@@ -1329,39 +1411,29 @@ __global__ __launch_bounds__(JAC_THREADS) void rc_jacobian_kernel(RcGeom g, int 
     rc_theta_offsets(nb, C, 2 * i, w1, b1);
     rc_theta_offsets(nb, C, 2 * i + 1, w2, b2);
     // conv2: dz2 = dX; input a1
-    for (int e = threadIdx.x; e < C * N; e += JAC_THREADS) bufA[e] = ra[e];
+    for (int e = threadIdx.x; e < C * N; e += JAC_THREADS) bA[e] = ra[e];
     __syncthreads();
-    jac_conv_grad_w<C>(bufX, bufA, C, g, Orow, w2, b2);
-    __syncthreads();
-    // da1 -> dz1 = da1 * gelu'(z1), into bufB
-    {
-      float din[C];
-      for (int q = threadIdx.x; q < N; q += JAC_THREADS) {
-        jac_conv_bwd_in<C>(bufX, C, theta + w2, g, q, din);
-#pragma unroll
-        for (int ci = 0; ci < C; ++ci) bufB[ci * N + q] = din[ci] * gelu_grad_f(rz[ci * N + q], gelu_exact);
-      }
-    }
+    jac_conv_grad_w_mfma<C>(bX, bA, C, nbr, N, NMAX, Orow, w2, b2);
+    // da1 -> dz1 = da1 * gelu'(z1), into bB
+    jac_conv_bwd_in_mfma<C>(bX, theta + w2, nbr, N, NMAX, [&](int ci, int q, float v) {
+      bB[ci * N + q] = v * gelu_grad_f(rz[ci * N + q], gelu_exact);
+    });
     __syncthreads();
     // conv1: input y_i
-    for (int e = threadIdx.x; e < cin * N; e += JAC_THREADS) bufA[e] = ry[e];
+    for (int e = threadIdx.x; e < cin * N; e += JAC_THREADS) bA[e] = ry[e];
     __syncthreads();
-    jac_conv_grad_w<C>(bufB, bufA, cin, g, Orow, w1, b1);
+    jac_conv_grad_w_mfma<C>(bB, bA, cin, nbr, N, NMAX, Orow, w1, b1);
     __syncthreads();
     if (i > 0) {
-      float din[C];
       const float sc = rsqrtf((float)(i + 1));
-      for (int q = threadIdx.x; q < N; q += JAC_THREADS) {
-        jac_conv_bwd_in<C>(bufB, C, theta + w1, g, q, din);
-#pragma unroll
-        for (int ci = 0; ci < C; ++ci) {
-          float xin = rx[ci * N + q] * sc;
-          bufA[ci * N + q] = din[ci] * gelu_grad_f(xin, gelu_exact) * sc + bufX[ci * N + q];
-        }
-      }
+      jac_conv_bwd_in_mfma<C>(bB, theta + w1, nbr, N, NMAX, [&](int ci, int q, float v) {
+        const float xin = rx[ci * N + q] * sc;
+        bA[ci * N + q] = v * gelu_grad_f(xin, gelu_exact) * sc + bX[ci * N + q];
+      });
       __syncthreads();
-      for (int e = threadIdx.x; e < C * N; e += JAC_THREADS) bufX[e] = bufA[e];
-      __syncthreads();
+      float* t = bX;
+      bX = bA;
+      bA = t;
     }
   }
 }

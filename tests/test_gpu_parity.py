@@ -235,11 +235,16 @@ def test_resconv_eloc_j1j2(precision, tol, rng):
     assert max_rel(e.cpu().numpy(), ref) < tol
 
 
-def test_resconv_jacobian(rng):
-    lat, model, spec, theta = _resconv(6, 2, 16)
-    s = rand_spins(rng, 5, 36)
+@pytest.mark.parametrize("L,nb,C,gelu", [(6, 2, 16, "tanh"), (16, 3, 16, "tanh"), (5, 2, 32, "tanh"),
+                                         (20, 2, 32, "tanh"), (6, 2, 16, "erf"), (3, 2, 16, "tanh")])
+def test_resconv_jacobian(L, nb, C, gelu, rng):
+    """Per-sample gradients (FP32 MFMA GEMMs) against the float64 oracle;
+    5 x 5 and 3 x 3 have partial 16-site tiles and a site count that is not a
+    multiple of 4."""
+    lat, model, spec, theta = _resconv(L, nb, C, gelu=gelu)
+    s = rand_spins(rng, 5, L * L)
     O = model.jacobian(to_dev(s)).cpu().numpy()
-    ref = om.resconv_jacobian(spec, theta, s)
+    ref = om.resconv_jacobian(spec, theta, s, approximate=(gelu == "tanh"))
     assert np.max(np.abs(O - ref)) < 1e-5 * max(1.0, np.max(np.abs(ref)))
 
 
