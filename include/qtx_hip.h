@@ -231,6 +231,18 @@ int qtx_colmean(void* stream, int o_dtype, const void* O, int64_t ns, int64_t np
  * r nullable (= 1). */
 int qtx_ebar(void* stream, const double* eloc, const double* r, int64_t ns, double* ebar, double* out2);
 
+/* y = beta y + alpha Obar v, Obar = (O - mean) sqrt(r/ns) formed on the fly
+ * (the Obar @ last_step / Obar @ m products of the momentum optimizers,
+ * optimizer/sr.py:249, :320, :411).  r nullable (= 1); beta = 0 ignores y's
+ * contents. */
+int qtx_obar_matvec(void* stream, int o_dtype, const void* O, const double* mean, const double* r, int64_t ns,
+                    int64_t np, double alpha, const double* v, double beta, double* y);
+
+/* O[:, k] *= d[k] in place (in o_dtype), and mean[k] *= d[k] when mean is
+ * given: the column scaling Obar /= V[None, :] of optimizer/sr.py:327, :416,
+ * applied to O and its mean because Obar is formed inside the solve. */
+int qtx_scale_columns(void* stream, int o_dtype, void* O, double* mean, int64_t ns, int64_t np, const double* d);
+
 /* step = solver(Obar, Ebar) with Obar = (O - mean) sqrt(r/ns) formed on the
  * fly.  ns < np: MinSR, T = Obar Obar^T (ns x ns); else SR, S = Obar^T Obar.
  * mode CHOL_SHIFT: shift = rshift * trace + ashift (solver.py:50-91);

@@ -108,3 +108,70 @@ def update(theta, step, dtype=None):
         return None
     dtype = np.asarray(theta).dtype if dtype is None else dtype
     return (np.asarray(theta) + (-step).astype(dtype)).astype(dtype)
+
+
+# ---------------------------------------------------------------- momentum SR variants
+# Each takes the SR solve as a callable solver(Obar, Ebar) -> step (sr.py:90-113
+# with the real, imaginary-time branch: step = solver(Obar, Ebar)).
+
+class Spring:
+    """``SPRING.solve`` (sr.py:247-255): Ebar -= mu Obar last; step =
+    solve(Obar, Ebar) + mu last; last = step."""
+
+    def __init__(self, nparams, solver, mu=0.9):
+        self.solver, self.mu = solver, mu
+        self.last = np.zeros(nparams)
+
+    def solve(self, Obar, Ebar):
+        Ebar = Ebar - self.mu * (Obar @ self.last)
+        step = self.solver(Obar, Ebar) + self.mu * self.last
+        self.last = step
+        return step
+
+
+class March:
+    """``MARCH.solve`` (sr.py:321-341): second-moment scaling V of the
+    parameter changes, bias-corrected, V**0.25 + 1e-8 (1 while V is all ~0)."""
+
+    def __init__(self, nparams, solver, mu=0.95, beta=0.995):
+        self.solver, self.mu, self.beta = solver, mu, beta
+        self.last = np.zeros(nparams)
+        self.V = np.zeros(nparams)
+        self.t = 0
+
+    def solve(self, Obar, Ebar):
+        self.t += 1
+        Ebar = Ebar - self.mu * (Obar @ self.last)
+        if np.allclose(self.V, 0):
+            V = np.ones_like(self.V)
+        else:
+            V = (self.V / (1 - self.beta ** self.t)) ** 0.25 + 1e-8
+        step = self.solver(Obar / V[None, :], Ebar)
+        step = step / V + self.mu * self.last
+        self.V = self.beta * self.V + (1 - self.beta) * np.abs(step - self.last) ** 2
+        self.last = step
+        return step
+
+
+class AdamSR:
+    """``AdamSR.solve`` (sr.py:405-421): a plain SR step g feeds Adam moments
+    m, v (bias-corrected); the step solves for the residual Ebar - Obar m in
+    the metric scaled by V = v**0.25 + 1e-8."""
+
+    def __init__(self, nparams, solver, mu=0.95, beta=0.995):
+        self.solver, self.mu, self.beta = solver, mu, beta
+        self.m = np.zeros(nparams)
+        self.v = np.zeros(nparams)
+        self.t = 0
+
+    def solve(self, Obar, Ebar):
+        self.t += 1
+        g = self.solver(Obar, Ebar)
+        self.m = self.mu * self.m + (1 - self.mu) * g
+        self.v = self.beta * self.v + (1 - self.beta) * np.abs(g) ** 2
+        m = self.m / (1 - self.mu ** self.t)
+        v = self.v / (1 - self.beta ** self.t)
+        V = v ** 0.25 + 1e-8
+        Ebar = Ebar - Obar @ m
+        step = self.solver(Obar / V[None, :], Ebar)
+        return step / V + m

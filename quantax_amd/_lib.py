@@ -82,6 +82,8 @@ PROTOS = {
     "qtx_resconv_jacobian": (C.c_int, [vp, C.POINTER(QtxResConvDesc), vp, vp, i64, vp, vp, sz]),
     "qtx_colmean": (C.c_int, [vp, C.c_int, vp, i64, i64, vp]),
     "qtx_ebar": (C.c_int, [vp, vp, vp, i64, vp, vp]),
+    "qtx_obar_matvec": (C.c_int, [vp, C.c_int, vp, vp, vp, i64, i64, C.c_double, vp, C.c_double, vp]),
+    "qtx_scale_columns": (C.c_int, [vp, C.c_int, vp, vp, i64, i64, vp]),
     "qtx_minsr_ws_bytes": (sz, [i64, i64, C.c_int, C.c_int]),
     "qtx_minsr_solve": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, i64, i64, C.c_int, C.c_int, f64, f64, f64, f64, vp,
                                   vp, sz]),

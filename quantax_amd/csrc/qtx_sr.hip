@@ -436,6 +436,41 @@ __global__ __launch_bounds__(256) void obar_t_y_kernel(const TI* __restrict__ O,
   atomicAdd(x + k, acc);
 }
 
+// y_s = beta y_s + alpha sqrt(r_s / ns) sum_k (O_sk - mean_k) v_k: the product
+// Obar v, one wave per sample row (FP64 accumulation)
+template <typename TI>
+__global__ __launch_bounds__(256) void obar_matvec_kernel(const TI* __restrict__ O, const double* __restrict__ mean,
+                                                          const double* __restrict__ r, int64_t ns, int64_t np,
+                                                          double inv_ns, double alpha, const double* __restrict__ v,
+                                                          double beta, double* __restrict__ y) {
+  const int lane = threadIdx.x % WAVE;
+  const int64_t s = (int64_t)blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE;
+  if (s >= ns) return;
+  const TI* row = O + s * np;
+  double a0 = 0.0, a1 = 0.0;
+  int64_t k = lane;
+  for (; k + WAVE < np; k += 2 * WAVE) {
+    a0 += ((double)row[k] - mean[k]) * v[k];
+    a1 += ((double)row[k + WAVE] - mean[k + WAVE]) * v[k + WAVE];
+  }
+  if (k < np) a0 += ((double)row[k] - mean[k]) * v[k];
+  const double acc = wave_sum(a0 + a1);
+  if (lane == 0) y[s] = (beta == 0.0 ? 0.0 : beta * y[s]) + alpha * sqrt((r ? r[s] : 1.0) * inv_ns) * acc;
+}
+
+// O[:, k] *= d_k (grid-stride over the ns * np elements), mean_k *= d_k
+template <typename TI>
+__global__ void scale_columns_kernel(TI* __restrict__ O, int64_t ns, int64_t np, const double* __restrict__ d) {
+  const int64_t total = ns * np;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x)
+    O[e] = (TI)((double)O[e] * d[e % np]);
+}
+
+__global__ void mul_f64_kernel(double* __restrict__ x, const double* __restrict__ d, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] *= d[i];
+}
+
 // single-block: scalars[0] = trace, then T_ii += rshift * trace + ashift
 __global__ __launch_bounds__(1024) void shift_diag_kernel(double* __restrict__ T, int64_t n, double rshift,
                                                           double ashift, double* __restrict__ scalars) {
@@ -1468,6 +1503,36 @@ int qtx_colmean(void* stream, int o_dtype, const void* O, int64_t ns, int64_t np
   else
     hipLaunchKernelGGL(colsum_kernel<double>, g, dim3(256), 0, st, (const double*)O, ns, np, rpc, mean);
   hipLaunchKernelGGL(scale_f64_kernel, dim3(nblk(np, 256)), dim3(256), 0, st, mean, np, 1.0 / (double)ns);
+  QTX_LAUNCH_CHECK();
+  return QTX_OK;
+}
+
+int qtx_obar_matvec(void* stream, int o_dtype, const void* O, const double* mean, const double* r, int64_t ns,
+                    int64_t np, double alpha, const double* v, double beta, double* y) {
+  QTX_REQUIRE(o_dtype == QTX_F32 || o_dtype == QTX_F64, "o_dtype must be QTX_F32 or QTX_F64");
+  QTX_REQUIRE(ns > 0 && np > 0 && O && mean && v && y, "bad arguments");
+  hipStream_t st = as_stream(stream);
+  dim3 g(nblk(ns, 256 / WAVE));
+  if (o_dtype == QTX_F32)
+    hipLaunchKernelGGL(obar_matvec_kernel<float>, g, dim3(256), 0, st, (const float*)O, mean, r, ns, np,
+                       1.0 / (double)ns, alpha, v, beta, y);
+  else
+    hipLaunchKernelGGL(obar_matvec_kernel<double>, g, dim3(256), 0, st, (const double*)O, mean, r, ns, np,
+                       1.0 / (double)ns, alpha, v, beta, y);
+  QTX_LAUNCH_CHECK();
+  return QTX_OK;
+}
+
+int qtx_scale_columns(void* stream, int o_dtype, void* O, double* mean, int64_t ns, int64_t np, const double* d) {
+  QTX_REQUIRE(o_dtype == QTX_F32 || o_dtype == QTX_F64, "o_dtype must be QTX_F32 or QTX_F64");
+  QTX_REQUIRE(ns > 0 && np > 0 && O && d, "bad arguments");
+  hipStream_t st = as_stream(stream);
+  const unsigned grid = (unsigned)std::min<int64_t>((ns * np + 255) / 256, kGridCap);
+  if (o_dtype == QTX_F32)
+    hipLaunchKernelGGL(scale_columns_kernel<float>, dim3(grid), dim3(256), 0, st, (float*)O, ns, np, d);
+  else
+    hipLaunchKernelGGL(scale_columns_kernel<double>, dim3(grid), dim3(256), 0, st, (double*)O, ns, np, d);
+  if (mean) hipLaunchKernelGGL(mul_f64_kernel, dim3(nblk(np, 256)), dim3(256), 0, st, mean, d, np);
   QTX_LAUNCH_CHECK();
   return QTX_OK;
 }

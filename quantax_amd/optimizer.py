@@ -7,7 +7,10 @@ get_step, energy, VarE), the solver factories of
 (mode, shifts, tolerances, Gram precision) executed by ``qtx_minsr_solve``:
 Ns < Np selects MinSR (T = Obar Obar^T, Ns x Ns), else SR (S = Obar^T Obar),
 exactly like the ``auto_*`` factories.  Obar is formed on the fly on the
-device; nothing of size Ns x Np is materialised beyond O itself.
+device; nothing of size Ns x Np is materialised beyond O itself.  The
+momentum variants SPRING, MARCH and AdamSR (``sr.py:198-429``) reuse the same
+solve, with Obar v products and the column scaling of Obar done by
+``qtx_obar_matvec`` and ``qtx_scale_columns``.
 """
 from __future__ import annotations
 
@@ -184,6 +187,138 @@ class SR(QNGD):
         _lib.call("qtx_ebar", _lib.stream(), eloc.data_ptr(), _lib.ptr(r), ns, ebar.data_ptr(), out2.data_ptr())
         self._energy, self._VarE = out2[0], out2[1]
         return ebar
+
+
+class _MomentumSR(SR):
+    """Shared plumbing of the momentum variants: ``get_step`` forms E_loc and
+    O as SR does and hands (O, mean, Ebar) to ``solve``, which may scale O's
+    columns in place (O is recomputed every step)."""
+
+    _saved = ()
+
+    def _vec(self):
+        return torch.zeros(self._state.nparams, dtype=torch.float64, device="cuda")
+
+    def get_step(self, samples) -> torch.Tensor:
+        ebar = self.get_Ebar(samples)
+        O, mean = self.get_Obar(samples)
+        r = None if samples.reweight_factor is None or self._trivial_r(samples) else samples.reweight_factor
+        return self.solve(O, mean, ebar, r)
+
+    def _sr(self, O, mean, ebar, r):
+        ns, np_ = O.shape
+        need = int(_lib.lib().qtx_minsr_ws_bytes(ns, np_, self._solver.mode, _GRAM[self._solver.gram]))
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = torch.empty(need, dtype=torch.uint8, device=O.device)
+        return solve_step(O, mean, ebar, self._solver, r, self._ws)
+
+    @staticmethod
+    def _odt(O):
+        return _lib.QTX_F32 if O.dtype == torch.float32 else _lib.QTX_F64
+
+    def _obar_v(self, O, mean, r, alpha, v, ebar):
+        """ebar + alpha Obar v (a new vector)."""
+        y = ebar.clone()
+        _lib.call("qtx_obar_matvec", _lib.stream(), self._odt(O), O.data_ptr(), mean.data_ptr(), _lib.ptr(r),
+                  O.shape[0], O.shape[1], float(alpha), v.data_ptr(), 1.0, y.data_ptr())
+        return y
+
+    def _scale_cols(self, O, mean, d):
+        _lib.call("qtx_scale_columns", _lib.stream(), self._odt(O), O.data_ptr(), mean.data_ptr(), O.shape[0],
+                  O.shape[1], d.data_ptr())
+
+    def save(self, file) -> None:
+        """The optimizer's internal quantities (``sr.py:257-262`` and alike), as
+        an .npz of this package (not the equinox leaf format)."""
+        np.savez(file, **{k: (v.cpu().numpy() if torch.is_tensor(v) else np.asarray(v))
+                          for k, v in ((k, getattr(self, k)) for k in self._saved)})
+
+    def _load(self, file) -> None:
+        with np.load(file, allow_pickle=False) as z:
+            for k in self._saved:
+                v = getattr(self, k)
+                setattr(self, k, torch.from_numpy(z[k]).to(v.device) if torch.is_tensor(v) else type(v)(z[k]))
+
+
+class SPRING(_MomentumSR):
+    """SR with momentum (``sr.py:198-262``): Ebar -= mu Obar last, step =
+    solve(Obar, Ebar) + mu last.  With the default mu = 0.9 the learning rate
+    should be about 1/5 of SR's."""
+
+    _saved = ("_mu", "_last_step")
+
+    def __init__(self, state, hamiltonian, imag_time: bool = True, solver: Optional[Solver] = None,
+                 mu: float = 0.9, file=None):
+        super().__init__(state, hamiltonian, imag_time, solver)
+        self._mu = float(mu)
+        self._last_step = self._vec()
+        if file is not None:
+            self._load(file)
+
+    def solve(self, O, mean, ebar, r=None) -> torch.Tensor:
+        ebar = self._obar_v(O, mean, r, -self._mu, self._last_step, ebar)
+        step = self._sr(O, mean, ebar, r) + self._mu * self._last_step
+        self._last_step = step.clone()
+        return step
+
+
+class MARCH(_MomentumSR):
+    """SR with first and second momenta of the parameter changes
+    (``sr.py:266-349``).  Defaults mu = 0.95, beta = 0.995; learning rate
+    about 1/5 of SR's."""
+
+    _saved = ("_mu", "_beta", "_last_step", "_V", "_t")
+
+    def __init__(self, state, hamiltonian, imag_time: bool = True, solver: Optional[Solver] = None,
+                 mu: float = 0.95, beta: float = 0.995, file=None):
+        super().__init__(state, hamiltonian, imag_time, solver)
+        self._mu, self._beta = float(mu), float(beta)
+        self._last_step = self._vec()
+        self._V = self._vec()
+        self._t = 0
+        if file is not None:
+            self._load(file)
+
+    def solve(self, O, mean, ebar, r=None) -> torch.Tensor:
+        self._t += 1
+        ebar = self._obar_v(O, mean, r, -self._mu, self._last_step, ebar)
+        if torch.allclose(self._V, torch.zeros_like(self._V)):
+            V = torch.ones_like(self._V)
+        else:
+            V = (self._V / (1 - self._beta ** self._t)) ** 0.25 + 1e-8
+        self._scale_cols(O, mean, 1.0 / V)
+        step = self._sr(O, mean, ebar, r) / V + self._mu * self._last_step
+        self._V = self._beta * self._V + (1 - self._beta) * (step - self._last_step) ** 2
+        self._last_step = step.clone()
+        return step
+
+
+class AdamSR(_MomentumSR):
+    """Adam moments of the SR step (``sr.py:352-429``): two solves per step."""
+
+    _saved = ("_mu", "_beta", "_m", "_v", "_t")
+
+    def __init__(self, state, hamiltonian, imag_time: bool = True, solver: Optional[Solver] = None,
+                 mu: float = 0.95, beta: float = 0.995, file=None):
+        super().__init__(state, hamiltonian, imag_time, solver)
+        self._mu, self._beta = float(mu), float(beta)
+        self._m = self._vec()
+        self._v = self._vec()
+        self._t = 0
+        if file is not None:
+            self._load(file)
+
+    def solve(self, O, mean, ebar, r=None) -> torch.Tensor:
+        self._t += 1
+        g = self._sr(O, mean, ebar, r)
+        self._m = self._mu * self._m + (1 - self._mu) * g
+        self._v = self._beta * self._v + (1 - self._beta) * g ** 2
+        m = self._m / (1 - self._mu ** self._t)
+        v = self._v / (1 - self._beta ** self._t)
+        V = v ** 0.25 + 1e-8
+        ebar = self._obar_v(O, mean, r, -1.0, m, ebar)
+        self._scale_cols(O, mean, 1.0 / V)
+        return self._sr(O, mean, ebar, r) / V + m
 
 
 class Euler:

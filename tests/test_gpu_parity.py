@@ -289,7 +289,8 @@ def test_minsr_solve(gram, tol, mode, shape, rng):
     Ebar, Em, Ev = osr.get_Ebar(E)
     eb = torch.empty(ns, dtype=torch.float64, device=DEV)
     out2 = torch.empty(2, dtype=torch.float64, device=DEV)
-    _lib.call("qtx_ebar", _lib.stream(), to_dev(E).data_ptr(), None, ns, eb.data_ptr(), out2.data_ptr())
+    Ed = to_dev(E)  # referenced until the call is done
+    _lib.call("qtx_ebar", _lib.stream(), Ed.data_ptr(), None, ns, eb.data_ptr(), out2.data_ptr())
     assert max_rel(eb.cpu().numpy(), Ebar) < 1e-12
     assert abs(out2[0].item() - Em) < 1e-12 * abs(Em) and abs(out2[1].item() - Ev) < 1e-12 * abs(Ev)
     Obar, Omean = osr.get_Obar(O.astype(np.float64))
@@ -314,7 +315,8 @@ def _solve_inputs(O, E):
     _lib.call("qtx_colmean", _lib.stream(), _lib.QTX_F32, Od.data_ptr(), ns, np_, mean.data_ptr())
     eb = torch.empty(ns, dtype=torch.float64, device=DEV)
     out2 = torch.empty(2, dtype=torch.float64, device=DEV)
-    _lib.call("qtx_ebar", _lib.stream(), to_dev(E).data_ptr(), None, ns, eb.data_ptr(), out2.data_ptr())
+    Ed = to_dev(E)  # referenced until the call is done
+    _lib.call("qtx_ebar", _lib.stream(), Ed.data_ptr(), None, ns, eb.data_ptr(), out2.data_ptr())
     return Od, mean, eb
 
 
