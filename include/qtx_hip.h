@@ -309,6 +309,48 @@ int qtx_minsr_last_path(int* path, int* lanczos_steps, double* lambda_max, int* 
 int qtx_apply_update(void* stream, int dtype, void* theta, const double* step, double lr, int64_t np,
                      int32_t* applied);
 
+/* ---------------------------------------------------------------- symmetry projection
+ * Images of each sample under a symmetry group, Symmetry.get_symm_spins
+ * (symmetry/symmetry.py:324-341): out[s * nimg + g] = s[perm[g]] for g < nperm
+ * and -s[perm[g - nperm]] after them when z2 != 0 (nimg = nperm * (z2 ? 2 : 1));
+ * perm: (nperm, N) int32 site indices. */
+int qtx_symm_images(void* stream, const int8_t* s, int64_t ns, int32_t N, const int32_t* perm, int32_t nperm,
+                    int32_t z2, int8_t* out);
+
+/* Symmetry.symmetrize (symmetry.py:343-392) in log form: psi_sym(s) =
+ * sum_g coef[g] sign_g exp(logabs[s * nimg + g]) -> (sign, logabs_out) per
+ * sample; coef[g] = chi_g chi_0 / nimg; sign_in nullable (= +1).  w (nullable):
+ * the weights coef_g psi_g / psi_sym of the images in d log|psi_sym| / d theta
+ * (the per-copy vjp sum of variational.py:436-491). */
+int qtx_symmetrize(void* stream, const double* logabs, const double* sign_in, const double* coef, int64_t ns,
+                   int32_t nimg, double* sign, double* logabs_out, double* w);
+
+/* O[s] = sum_g w[s, g] Oimg[s * nimg + g] (rows of np, o_dtype; FP64 sums):
+ * the symmetrised Jacobian from the images' Jacobians. */
+int qtx_rows_combine(void* stream, int o_dtype, const void* Oimg, const double* w, int64_t ns, int32_t nimg,
+                     int64_t np, void* O);
+
+/* One proposal per chain from this step's Philox words (words: ns x 4 uint32,
+ * the slice of one step), LocalFlip / SpinExchange as in the fused Metropolis
+ * kernels (sampler/common_samplers.py:14-162): s_prop = s with the move applied,
+ * updated[c] = 0 when the move leaves s unchanged.  N <= 1024. */
+int qtx_propose(void* stream, int kind, const int32_t* nbr, int32_t width, int32_t hopping, const uint32_t* words,
+                const int8_t* s, int64_t ns, int32_t N, int8_t* s_prop, int32_t* updated);
+
+/* Metropolis._update (sampler/metropolis.py:291-322) for the proposals of
+ * qtx_propose: accept iff reweight (lp' - lp) > log1p(-u) or lp = -inf, u from
+ * the same words; copies s_prop into s, lp_prop into lp (and sign_prop into
+ * sign, both nullable); counts acceptances and |d - log1p(-u)| < tie_tol. */
+int qtx_accept(void* stream, const uint32_t* words, const int8_t* s_prop, const double* lp_prop, const double* sg_prop,
+               const int32_t* updated, int64_t ns, int32_t N, double reweight, double tie_tol, int8_t* s, double* lp,
+               double* sg, int32_t* n_accept, int32_t* n_near);
+
+/* E_loc(s) = diag(s) + sum_{k in [off[s], off[s+1])} H_k (sign_k / sign_s)
+ * exp(logabs_k - logabs_s) over compacted connected configurations
+ * (operator/operator.py:168-184, 510-562); sign_c, sign nullable (= +1). */
+int qtx_conn_combine(void* stream, const double* diag, const int64_t* off, const double* hc, const double* sign_c,
+                     const double* logabs_c, const double* sign, const double* logabs, int64_t ns, double* eloc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -84,6 +84,12 @@ PROTOS = {
     "qtx_ebar": (C.c_int, [vp, vp, vp, i64, vp, vp]),
     "qtx_obar_matvec": (C.c_int, [vp, C.c_int, vp, vp, vp, i64, i64, C.c_double, vp, C.c_double, vp]),
     "qtx_scale_columns": (C.c_int, [vp, C.c_int, vp, vp, i64, i64, vp]),
+    "qtx_symm_images": (C.c_int, [vp, vp, i64, C.c_int32, vp, C.c_int32, C.c_int32, vp]),
+    "qtx_symmetrize": (C.c_int, [vp, vp, vp, vp, i64, C.c_int32, vp, vp, vp]),
+    "qtx_rows_combine": (C.c_int, [vp, C.c_int, vp, vp, i64, C.c_int32, i64, vp]),
+    "qtx_propose": (C.c_int, [vp, C.c_int, vp, C.c_int32, C.c_int32, vp, vp, i64, C.c_int32, vp, vp]),
+    "qtx_accept": (C.c_int, [vp, vp, vp, vp, vp, vp, i64, C.c_int32, C.c_double, C.c_double, vp, vp, vp, vp, vp]),
+    "qtx_conn_combine": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, i64, vp]),
     "qtx_minsr_ws_bytes": (sz, [i64, i64, C.c_int, C.c_int]),
     "qtx_minsr_solve": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, i64, i64, C.c_int, C.c_int, f64, f64, f64, f64, vp,
                                   vp, sz]),

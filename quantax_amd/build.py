@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libqtxhip.so")
-SOURCES = ["qtx_core.hip", "qtx_sr.hip", "qtx_resconv.hip", "qtx_serve.hip"]
+SOURCES = ["qtx_core.hip", "qtx_sr.hip", "qtx_resconv.hip", "qtx_serve.hip", "qtx_symm.hip"]
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 

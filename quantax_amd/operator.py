@@ -174,9 +174,10 @@ class Operator:
         _lib.call("qtx_ham_diag", _lib.stream(), self.device_tables(), s.data_ptr(), s.shape[0], out.data_ptr())
         return out
 
-    def connected(self, s: torch.Tensor):
+    def connected(self, s: torch.Tensor, with_offsets: bool = False):
         """Compacted connected configurations (segment, candidate, s', H), the
-        counterpart of ``_apply_off_diag`` + ``_get_conn`` (operator.py:96-165)."""
+        counterpart of ``_apply_off_diag`` + ``_get_conn`` (operator.py:96-165);
+        with_offsets: also the (ns + 1) per-sample offsets into them."""
         s = s.contiguous()
         ns, N = s.shape
         H = self.device_tables()
@@ -192,6 +193,8 @@ class Operator:
         if K:
             _lib.call("qtx_conn_list", _lib.stream(), H, s.data_ptr(), ns, off.data_ptr(), seg.data_ptr(),
                       cand.data_ptr(), sc.data_ptr(), hc.data_ptr())
+        if with_offsets:
+            return seg, cand, sc, hc, off
         return seg, cand, sc, hc
 
     def Oloc(self, state, samples) -> torch.Tensor:
@@ -203,9 +206,35 @@ class Operator:
         else:
             s = torch.as_tensor(samples, device=_lib.device()).to(torch.int8)
             psi = state(s)
-        eloc, nconn = state.model.eloc(self, s, psi.logabs)
+        if getattr(state, "is_symmetric", False):
+            eloc, nconn = self._oloc_generic(state, s, psi)
+        else:
+            eloc, nconn = state.model.eloc(self, s, psi.logabs)
         self._connectivity = nconn
         return eloc
+
+    def _oloc_generic(self, state, s, psi, chunk: int = 1 << 18):
+        """Materialised connected configurations, psi of each (the state's
+        own forward, e.g. symmetrised), then the per-sample sum
+        (``operator.py:168-184``)."""
+        s = s.contiguous()
+        ns = s.shape[0]
+        diag = self.apply_diag(s)
+        _, _, sc, hc, off = self.connected(s, with_offsets=True)
+        K = sc.shape[0]
+        sg_c = torch.empty(K, dtype=torch.float64, device=s.device)
+        la_c = torch.empty(K, dtype=torch.float64, device=s.device)
+        for k0 in range(0, K, chunk):
+            p = state(sc[k0: k0 + chunk])
+            sg_c[k0: k0 + chunk] = p.sign
+            la_c[k0: k0 + chunk] = p.logabs
+        sg = psi.sign.to(torch.float64).contiguous()
+        la = psi.logabs.contiguous()
+        eloc = torch.empty(ns, dtype=torch.float64, device=s.device)
+        _lib.call("qtx_conn_combine", _lib.stream(), diag.data_ptr(), off.data_ptr(), _lib.ptr(hc if K else None),
+                  sg_c.data_ptr() if K else None, la_c.data_ptr() if K else None, sg.data_ptr(), la.data_ptr(), ns,
+                  eloc.data_ptr())
+        return eloc, (off[1:] - off[:-1]).to(torch.int32)
 
     def expectation(self, state, samples, return_var: bool = False):
         """``operator.py:564-595``."""

@@ -97,9 +97,32 @@ class Metropolis:
     def reset(self) -> None:
         """Reset the chains to ``initial_spins`` (or random ones) and thermalise."""
         self._spins = self._initial() if self._initial_spins is None else self._initial_spins.clone()
-        self._logabs = self._state.model.logpsi(self._spins)
+        psi = self._state(self._spins)
+        self._sign, self._logabs = psi.sign.contiguous(), psi.logabs.contiguous()
         if self._thermal_steps > 0:
             self.sweep(self._thermal_steps)
+
+    @property
+    def _generic(self) -> bool:
+        """Symmetrised states (and the tests of this path) step through
+        propose -> images -> forward -> project -> accept."""
+        return self._state.is_symmetric or getattr(self, "force_generic", False)
+
+    def _generic_steps(self, words: torch.Tensor, nsteps: int) -> None:
+        st = self._state
+        ns, N = self._spins.shape
+        s_prop = torch.empty_like(self._spins)
+        upd = torch.empty(ns, dtype=torch.int32, device=self._spins.device)
+        for t in range(nsteps):
+            w = words[t]
+            _lib.call("qtx_propose", _lib.stream(), self.kind, _lib.ptr(self._nbr), self._width, self._hopping,
+                      w.data_ptr(), self._spins.data_ptr(), ns, N, s_prop.data_ptr(), upd.data_ptr())
+            psi = st(s_prop)
+            sg, la = psi.sign.contiguous(), psi.logabs.contiguous()
+            _lib.call("qtx_accept", _lib.stream(), w.data_ptr(), s_prop.data_ptr(), la.data_ptr(), sg.data_ptr(),
+                      upd.data_ptr(), ns, N, self._reweight, self._tie_tol, self._spins.data_ptr(),
+                      self._logabs.data_ptr(), self._sign.data_ptr(), self.n_accept.data_ptr(),
+                      self.n_near.data_ptr())
 
     def sweep(self, nsweeps: Optional[int] = None) -> Samples:
         nsteps = self._sweep_steps if nsweeps is None else int(nsweeps)
@@ -111,12 +134,18 @@ class Metropolis:
                       words.data_ptr())
             ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             ev0.record()
-            model.metropolis(self.kind, self._nbr, self._width, self._hopping, words, nsteps, self._spins,
-                             self._logabs, self.n_accept, self.n_near, self._reweight, self._tie_tol)
+            if self._generic:
+                self._generic_steps(words, nsteps)
+            else:
+                model.metropolis(self.kind, self._nbr, self._width, self._hopping, words, nsteps, self._spins,
+                                 self._logabs, self.n_accept, self.n_near, self._reweight, self._tie_tol)
             ev1.record()
             self.kernel_events.append((ev0, ev1))
         self._step += nsteps
         self.n_proposed += nsteps
+        if self._generic:
+            la = self._logabs.clone()
+            return Samples(self._spins.clone(), LogArray(self._sign.clone(), la), None, self._reweight_factor(la))
         uses_ref = self._state.use_ref and hasattr(model, "M")
         if uses_ref:
             # local-update drift check and replacement by the full forward (metropolis.py:201-213)

@@ -6,7 +6,10 @@ O = d log|psi| / d theta per sample in ravel_pytree order
 (``variational.py:424-511``); ``update`` applies theta <- theta - step and
 skips non-finite steps with a warning (``variational.py:558-579``).
 ``max_parallel`` is accepted for signature compatibility: the kernels tile
-samples internally, so results never depend on it.
+samples internally, so results never depend on it.  With a non-trivial
+``symm`` (``variational.py:261-323``, ``436-491``) the forward evaluates the
+model on every image and projects (``Symmetry.project``), and the Jacobian
+is the images' Jacobians weighted by c_g psi_g / psi_sym.
 """
 from __future__ import annotations
 
@@ -22,8 +25,7 @@ from .utils import LogArray
 
 class Variational:
     def __init__(self, model, param_file: Optional[str] = None, symm=None, max_parallel=None, use_ref: bool = True):
-        if symm is not None:
-            raise NotImplementedError("only the identity symmetry is supported")
+        self._symm = None if symm is None or symm.is_identity else symm
         self._model = model
         self._max_parallel = max_parallel
         self._use_ref = use_ref
@@ -54,14 +56,43 @@ class Variational:
     def use_ref(self) -> bool:
         return self._use_ref
 
+    @property
+    def symm(self):
+        from .symmetry import Identity
+
+        return Identity() if self._symm is None else self._symm
+
+    @property
+    def is_symmetric(self) -> bool:
+        return self._symm is not None
+
     def __call__(self, s) -> LogArray:
         s = torch.as_tensor(s, device=_lib.device()).to(torch.int8).reshape(-1, self.Nmodes)
-        la = self._model.logpsi(s)
-        return LogArray(torch.ones_like(la), la)
+        if self._symm is None:
+            la = self._model.logpsi(s)
+            return LogArray(torch.ones_like(la), la)
+        sg, la = self._symm.project(self._model.logpsi(self._symm.images(s)))
+        return LogArray(sg, la)
 
     def jacobian(self, fock_states) -> torch.Tensor:
         s = torch.as_tensor(fock_states, device=_lib.device()).to(torch.int8).reshape(-1, self.Nmodes)
-        return self._model.jacobian(s)
+        if self._symm is None:
+            return self._model.jacobian(s)
+        symm = self._symm
+        ns, nimg, np_ = s.shape[0], symm.nsymm, self.nparams
+        O = None
+        # images' Jacobians of at most ~4 GB at a time
+        chunk = max(1, min(ns, (4 << 30) // max(1, nimg * np_ * 8)))
+        for s0 in range(0, ns, chunk):
+            sc = s[s0: s0 + chunk]
+            img = symm.images(sc)
+            _, _, w = symm.project(self._model.logpsi(img), weights=True)
+            Oi = self._model.jacobian(img)
+            if O is None:
+                O = torch.empty((ns, np_), dtype=Oi.dtype, device=Oi.device)
+            _lib.call("qtx_rows_combine", _lib.stream(), _lib.QTX_F32 if Oi.dtype == torch.float32 else _lib.QTX_F64,
+                      Oi.data_ptr(), w.data_ptr(), sc.shape[0], nimg, np_, O[s0:].data_ptr())
+        return O
 
     def get_params_flatten(self) -> torch.Tensor:
         return self._model.theta.clone()

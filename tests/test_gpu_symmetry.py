@@ -1,0 +1,160 @@
+"""Symmetry projection on the device against the oracle (needs an MI355X):
+images, the projected log-amplitude and sign, the projected Jacobian (also
+against finite differences of the oracle's projected wavefunction), E_loc
+over projected amplitudes, and the generic Metropolis step (propose -> images
+-> forward -> project -> accept), which with the identity symmetry must
+reproduce the fused kernels' chains."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import models as om
+from oracle import operator as oop
+from oracle import symmetry as osym
+from tests.helpers import max_rel
+
+pytestmark = pytest.mark.gpu
+
+import quantax_amd as qtx  # noqa: E402
+
+SYMMS = {
+    "c4v_a1_z2": lambda: qtx.symmetry.C4v() @ qtx.symmetry.SpinInverse(1),
+    "c4v_b2_zm": lambda: qtx.symmetry.C4v(repr="B2") @ qtx.symmetry.SpinInverse(-1),
+    "trans_c4v": lambda: qtx.symmetry.TransND(sector=[0, 2]) @ qtx.symmetry.C4v(repr="B1"),
+    "flip_y": lambda: qtx.symmetry.Flip(axis=1, sector=1),
+}
+
+
+def _rbm(symm_key, M=24, seed=2, nup=None):
+    lat = qtx.sites.Square(4, Nparticles=None if nup is None else (nup, 16 - nup))
+    model = qtx.model.RBM_Dense(M, dtype=np.float64, init_std=0.2, seed=seed)
+    symm = SYMMS[symm_key]()
+    state = qtx.state.Variational(model, symm=symm)
+    W, b = om.rbm_unpack(state.get_params_flatten().cpu().numpy(), M, 16)
+    fn = osym.logpsi_sym_fn(lambda x: om.rbm_logpsi(W, b, x)[1], symm.perm, symm.character, symm.Z2_inversion)
+    return lat, model, state, symm, W, b, fn
+
+
+def _spins(rng, n, nup=None):
+    if nup is None:
+        return (2 * rng.integers(0, 2, (n, 16)) - 1).astype(np.int8)
+    out = -np.ones((n, 16), np.int8)
+    for i in range(n):
+        out[i, rng.permutation(16)[:nup]] = 1
+    return out
+
+
+def test_images_match_host(rng):
+    lat = qtx.sites.Square(4)
+    symm = qtx.symmetry.TransND() @ qtx.symmetry.C4v() @ qtx.symmetry.SpinInverse(1)
+    s = _spins(rng, 5)
+    img = symm.images(torch.from_numpy(s).cuda()).cpu().numpy().reshape(5, symm.nsymm, 16)
+    for i in range(5):
+        assert np.array_equal(img[i], symm.get_symm_spins(s[i]))
+
+
+@pytest.mark.parametrize("key", list(SYMMS))
+def test_logpsi_sym_rbm(key, rng):
+    """Configurations mapped to themselves by an element of character -1
+    have psi_sym = 0 exactly; the two sums leave it at 0 or at round-off of
+    either sign, so they are compared as 'vanishing' (|psi_sym| below 1e-10 of
+    the largest image amplitude) and everything else to 1e-12."""
+    lat, model, state, symm, W, b, fn = _rbm(key)
+    s = _spins(rng, 40)
+    psi = state(torch.from_numpy(s).cuda())
+    sg, la = fn(s)
+    big = np.array([om.rbm_logpsi(W, b, osym.images(x, symm.perm, symm.Z2_inversion))[1].max() for x in s])
+    dev_la = psi.logabs.cpu().numpy()
+    zero_ref = ~np.isfinite(la) | (la - big < np.log(1e-10))
+    zero_dev = ~np.isfinite(dev_la) | (dev_la - big < np.log(1e-10))
+    assert np.array_equal(zero_ref, zero_dev)
+    ok = ~zero_ref
+    assert np.array_equal(psi.sign.cpu().numpy()[ok], sg[ok])
+    assert max_rel(dev_la[ok], la[ok]) < 1e-12
+
+
+def test_logpsi_sym_resconv(rng):
+    lat = qtx.sites.Square(4)
+    model = qtx.model.ResConv(2, 16, precision="fp32", seed=3)
+    symm = qtx.symmetry.C4v(repr="A1") @ qtx.symmetry.SpinInverse(1)
+    state = qtx.state.Variational(model, symm=symm)
+    spec = om.ResConvSpec(2, 16, (4, 4))
+    theta = model.theta.cpu().numpy().astype(np.float64)
+    fn = osym.logpsi_sym_fn(lambda x: om.resconv_forward(spec, theta, x)[1], symm.perm, symm.character, 1)
+    s = _spins(rng, 12)
+    psi = state(torch.from_numpy(s).cuda())
+    sg, la = fn(s)
+    assert np.array_equal(psi.sign.cpu().numpy(), sg)
+    assert max_rel(psi.logabs.cpu().numpy(), la) < 1e-5
+
+
+@pytest.mark.parametrize("key", ["c4v_a1_z2", "trans_c4v"])
+def test_jacobian_sym_rbm(key, rng):
+    lat, model, state, symm, W, b, fn = _rbm(key, M=8)
+    s = _spins(rng, 6)
+    O = state.jacobian(torch.from_numpy(s).cuda()).cpu().numpy()
+    # oracle: weighted image Jacobians
+    c = osym.coefficients(symm.character, symm.Z2_inversion)
+    ref = np.zeros_like(O)
+    for i, x in enumerate(s):
+        img = osym.images(x, symm.perm, symm.Z2_inversion)
+        la = om.rbm_logpsi(W, b, img)[1]
+        wts = c * np.exp(la - la.max())
+        ref[i] = (wts / wts.sum()) @ om.rbm_jacobian(W, b, img)
+    assert np.max(np.abs(O - ref)) < 1e-11 * max(1.0, np.abs(ref).max())
+    # finite differences of log|psi_sym| (independent of the weighting formula)
+    theta = om.rbm_pack(W, b)
+    for k in rng.choice(theta.size, 6, replace=False):
+        e = np.zeros_like(theta)
+        e[k] = 1e-6
+        f = lambda th: osym.logpsi_sym_fn(lambda x: om.rbm_logpsi(*om.rbm_unpack(th, 8, 16), x)[1], symm.perm,
+                                           symm.character, symm.Z2_inversion)(s)[1]
+        fd = (f(theta + e) - f(theta - e)) / 2e-6
+        assert np.max(np.abs(O[:, k] - fd)) < 1e-6 * max(1.0, np.abs(fd).max())
+
+
+@pytest.mark.parametrize("key", ["c4v_a1_z2", "c4v_b2_zm"])
+def test_oloc_sym(key, rng):
+    lat, model, state, symm, W, b, fn = _rbm(key, nup=8)
+    H = qtx.operator.Heisenberg(J=[1, 0.5], n_neighbor=[1, 2], msr=True)
+    Ho = oop.heisenberg([4, 4], [1, 0.5], [1, 2], True)
+    s = _spins(rng, 20, nup=8)
+    sd = torch.from_numpy(s).cuda()
+    psi = state(sd)
+    e = H.Oloc(state, sd).cpu().numpy()
+    ref = oop.oloc(s, fn, Ho)
+    ok = np.isfinite(ref)
+    assert max_rel(e[ok], ref[ok]) < 1e-10
+
+
+@pytest.mark.parametrize("kind", ["flip", "exchange"])
+def test_generic_step_reproduces_fused_chains(kind):
+    """Identity symmetry through the generic path vs the fused kernels: same
+    Philox words, so the chains agree except where a proposal sits within
+    tie_tol of its threshold (the two paths round log psi differently)."""
+    qtx.sites.Square(4, Nparticles=(8, 8) if kind == "exchange" else None)
+    model = qtx.model.RBM_Dense(24, dtype=np.float64, init_std=0.3, seed=5)
+    state = qtx.state.Variational(model)
+    cls = qtx.sampler.LocalFlip if kind == "flip" else qtx.sampler.SpinExchange
+    a = cls(state, 128, thermal_steps=0, sweep_steps=64, seed=11, tie_tol=1e-9)
+    b = cls(state, 128, thermal_steps=0, sweep_steps=64, seed=11, tie_tol=1e-9)
+    b.force_generic = True
+    for _ in range(2):
+        sa, sb = a.sweep(), b.sweep()
+    clean = (a.n_near.cpu().numpy() == 0) & (b.n_near.cpu().numpy() == 0)
+    assert clean.sum() >= 120
+    assert np.array_equal(sa.spins.cpu().numpy()[clean], sb.spins.cpu().numpy()[clean])
+    assert max_rel(sa.psi.logabs.cpu().numpy()[clean], sb.psi.logabs.cpu().numpy()[clean]) < 1e-12
+    assert np.array_equal(a.n_accept.cpu().numpy()[clean], b.n_accept.cpu().numpy()[clean])
+
+
+def test_symmetric_sampler_tracks_psi(rng):
+    lat, model, state, symm, W, b, fn = _rbm("c4v_a1_z2", nup=8)
+    sampler = qtx.sampler.SpinExchange(state, 64, thermal_steps=32, sweep_steps=32, seed=3)
+    smp = sampler.sweep()
+    s = smp.spins.cpu().numpy()
+    assert np.all(s.sum(1) == 0)
+    sg, la = fn(s)
+    assert np.array_equal(smp.psi.sign.cpu().numpy(), sg)
+    assert max_rel(smp.psi.logabs.cpu().numpy(), la) < 1e-12
+    assert 0.0 < sampler.acceptance < 1.0
