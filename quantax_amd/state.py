@@ -117,7 +117,56 @@ class Variational:
         return True
 
     def save(self, file) -> None:
+        """The flat parameter vector as one .npy array (ravel_pytree order)."""
         np.save(file, self._model.theta.cpu().numpy(), allow_pickle=False)
 
     def load(self, file) -> None:
-        self._model.set_params(np.load(file, allow_pickle=False))
+        """Parameters from ``save`` above, or from a quantax checkpoint written by
+        ``Variational.save`` = ``equinox.tree_serialise_leaves`` (variational.py:
+        581-587, read at :162-163): see ``read_leaf_stream``."""
+        self._model.set_params(params_from_file(file, self.nparams))
+
+
+def read_leaf_stream(file) -> list:
+    """The arrays of a leaf stream as ``equinox.tree_serialise_leaves`` writes
+    it: one ``.npy`` record per array-like leaf (arrays, and bool/int/float
+    scalars), in pytree-flatten order, back to back; other leaves write
+    nothing.  Read with ``allow_pickle=False``."""
+    out = []
+    with open(file, "rb") if isinstance(file, (str, bytes)) or hasattr(file, "__fspath__") else _nullctx(file) as f:
+        while True:
+            head = f.read(6)
+            if len(head) < 6:
+                break
+            if head != b"\x93NUMPY":
+                raise ValueError("not a stream of .npy records")
+            f.seek(-6, 1)
+            out.append(np.lib.format.read_array(f, allow_pickle=False))
+    return out
+
+
+class _nullctx:
+    def __init__(self, f):
+        self.f = f
+
+    def __enter__(self):
+        return self.f
+
+    def __exit__(self, *a):
+        return False
+
+
+def params_from_file(file, nparams: int) -> np.ndarray:
+    """Flat parameters from a single-array file or an equinox leaf stream: the
+    floating-point leaves of rank >= 1 are the model's parameters, in ravel_pytree
+    order (integer and boolean scalars such as ``nblocks`` or ``holomorphic``
+    are configuration); their total size must equal nparams."""
+    leaves = read_leaf_stream(file)
+    if len(leaves) == 1 and leaves[0].size == nparams:
+        return leaves[0].reshape(-1)
+    params = [a for a in leaves if np.issubdtype(a.dtype, np.floating) and a.ndim >= 1]
+    total = sum(a.size for a in params)
+    if total != nparams:
+        raise ValueError(f"the file holds {total} floating-point parameters in {len(params)} arrays; "
+                         f"the model has {nparams}")
+    return np.concatenate([a.reshape(-1) for a in params])
