@@ -213,12 +213,14 @@ class Operator:
         self._connectivity = nconn
         return eloc
 
-    def _oloc_generic(self, state, s, psi, chunk: int = 1 << 18):
+    def _oloc_generic(self, state, s, psi):
         """Materialised connected configurations, psi of each (the state's
         own forward, e.g. symmetrised), then the per-sample sum
-        (``operator.py:168-184``)."""
+        (``operator.py:168-184``).  Connected configurations are evaluated in
+        chunks whose images stay near 1 GB."""
         s = s.contiguous()
         ns = s.shape[0]
+        chunk = max(1024, (1 << 30) // (state.symm.nsymm * s.shape[1]))
         diag = self.apply_diag(s)
         _, _, sc, hc, off = self.connected(s, with_offsets=True)
         K = sc.shape[0]

@@ -179,3 +179,24 @@ def test_heisenberg4x4_symmetric_rbm_sr():
     e = drv.energy.mean(20)
     e0 = -44.913932833715506
     assert abs(e - e0) / abs(e0) < 1e-2, e
+
+
+def test_jacobian_sym_resconv(rng):
+    """Projected Jacobian of a ResConv (FP32 MFMA per image) against the
+    oracle's image Jacobians weighted by c_g psi_g / psi_sym."""
+    lat = qtx.sites.Square(4)
+    model = qtx.model.ResConv(2, 16, precision="fp32", seed=4)
+    symm = qtx.symmetry.Rotation(np.pi / 2) @ qtx.symmetry.SpinInverse(1)
+    state = qtx.state.Variational(model, symm=symm)
+    spec = om.ResConvSpec(2, 16, (4, 4))
+    theta = model.theta.cpu().numpy().astype(np.float64)
+    s = _spins(rng, 4)
+    O = state.jacobian(torch.from_numpy(s).cuda()).cpu().numpy()
+    c = osym.coefficients(symm.character, symm.Z2_inversion)
+    ref = np.zeros_like(O, dtype=np.float64)
+    for i, x in enumerate(s):
+        img = osym.images(x, symm.perm, symm.Z2_inversion)
+        la = om.resconv_forward(spec, theta, img)[1]
+        wts = c * np.exp(la - la.max())
+        ref[i] = (wts / wts.sum()) @ om.resconv_jacobian(spec, theta, img)
+    assert np.max(np.abs(O - ref)) < 1e-5 * max(1.0, np.abs(ref).max())
