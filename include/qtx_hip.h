@@ -250,7 +250,8 @@ int qtx_scale_columns(void* stream, int o_dtype, void* O, double* mean, int64_t 
  *                  (solver.py:94-201, tol_snr = 0).  When c >= 1e-5 lambda_max it is
  *                  evaluated without eigenvectors: g(v) = v^5/(v^6 + c^6) =
  *                  (1/3) sum_k Re 1/(v - p_k), p_k = c exp(i pi (2k+1)/6), each complex
- *                  shifted system solved by Cholesky of (T - Re p)^2 + (Im p)^2 plus
+ *                  shifted system solved by Cholesky of (T - Re p)^2 + (Im p)^2 (the
+ *                  three factored together by this library's blocked Cholesky) plus
  *                  iterative refinement against T; lambda_max by Lanczos.  Smaller
  *                  cutoffs, or a failed factorisation, use rocSOLVER dsyevd.
  * mode EIGH_PINV_DIRECT: the same g(T) b through dsyevd always.
