@@ -200,3 +200,45 @@ def test_jacobian_sym_resconv(rng):
         wts = c * np.exp(la - la.max())
         ref[i] = (wts / wts.sum()) @ om.resconv_jacobian(spec, theta, img)
     assert np.max(np.abs(O - ref)) < 1e-5 * max(1.0, np.abs(ref).max())
+
+
+def _exact_sz0_probs(fn, L0, L1, nup):
+    import itertools
+    N = L0 * L1
+    basis = []
+    for ups in itertools.combinations(range(N), nup):
+        x = -np.ones(N, np.int8)
+        x[list(ups)] = 1
+        basis.append(x)
+    basis = np.array(basis)
+    sg, la = fn(basis)
+    p = np.exp(2 * (la - la.max()))
+    return basis, p / p.sum()
+
+
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_sampler_stationary_distribution(symmetric):
+    """3 x 4 lattice, Sz = 0 (924 configurations), RBM float64 with a flat
+    enough |psi|^2 that 1200 exchange proposals per chain mix: one sample per
+    chain from 16384 thermalised chains against |psi|^2 (chi-square); the
+    symmetric case samples |psi_sym|^2 through the generic step."""
+    rng = np.random.default_rng(7)
+    qtx.sites.Grid([3, 4], Nparticles=(6, 6))
+    model = qtx.model.RBM_Dense(12, dtype=np.float64, init_std=0.15, seed=9)
+    symm = (qtx.symmetry.Flip(axis=0) @ qtx.symmetry.SpinInverse(1)) if symmetric else None
+    state = qtx.state.Variational(model, symm=symm)
+    W, b = om.rbm_unpack(state.get_params_flatten().cpu().numpy(), 12, 12)
+    if symmetric:
+        fn = osym.logpsi_sym_fn(lambda x: om.rbm_logpsi(W, b, x)[1], symm.perm, symm.character, 1)
+    else:
+        fn = lambda x: om.rbm_logpsi(W, b, x)
+    basis, p = _exact_sz0_probs(fn, 3, 4, 6)
+    sampler = qtx.sampler.SpinExchange(state, 16384, thermal_steps=1200, seed=int(rng.integers(1 << 30)))
+    s = sampler.sweep().spins.cpu().numpy()
+    key = lambda a: (a > 0).astype(np.int64) @ (1 << np.arange(12))
+    idx = {int(k): i for i, k in enumerate(key(basis))}
+    counts = np.bincount([idx[int(k)] for k in key(s)], minlength=basis.shape[0])
+    expected = p * s.shape[0]
+    chi2 = np.sum((counts - expected) ** 2 / expected)
+    dof = basis.shape[0] - 1
+    assert chi2 < dof + 6 * np.sqrt(2 * dof), chi2
