@@ -195,6 +195,7 @@ def run_mode(cfg, mode, args, torch, qtx):
     out["eloc_ms"] = out["stages"]["eloc"]
     if mode == "mixed" and cfg["model"] == "resconv":
         if cfg["L"] == 16:
+            out["gram"] = gram_rate(last["O"], last["mean"], torch)
             out["precision"] = precision_report(cfg, state, H, solver, last, torch, qtx, args.precision_breakdown)
         else:
             out["gram"] = gram_report(last, solver, torch, qtx, opt._ws)
@@ -241,14 +242,11 @@ def precision_report(cfg, state, H, solver, last, torch, qtx, breakdown=False):
     return rep
 
 
-def gram_report(last, solver, torch, qtx, ws_solve):
-    """The large case: MFMA rate of the split-bf16 Gram entry point on the last
-    step's O, and the update-vector error of the mixed solve against the same
-    route with the FP64 Gram, both on the last step's O and E_loc (the errors of
-    the network precision are measured on the 16x16 workload)."""
+def gram_rate(O, mean, torch):
+    """MFMA rate of the split-bf16 Gram entry point (centring/split pass + MFMA
+    Gram) on a step's O: two launches timed by HIP events, the faster kept."""
     from quantax_amd import _lib
 
-    O, mean, eb = last["O"], last["mean"], last["ebar"]
     ns, np_ = O.shape
     g = _lib.QTX_GRAM_BF16X3
     nws = int(_lib.lib().qtx_gram_ws_bytes(ns, np_, g))
@@ -274,6 +272,16 @@ def gram_report(last, solver, torch, qtx, ws_solve):
            "flop_convention": "Ns^2 Np (SYRK-halved, one multiply-add = 2 flops over the lower triangle); "
                               "issue = 3 split products"}
     print(f"[bench] gram: {ms:.1f} ms, {rep['achieved_tflops']} TFLOP/s algorithmic", file=sys.stderr, flush=True)
+    return rep
+
+
+def gram_report(last, solver, torch, qtx, ws_solve):
+    """The large case: the Gram's MFMA rate on the last step's O and the
+    update-vector error of the mixed solve against the same route with the
+    FP64 Gram, both on the last step's O and E_loc (the errors of the network
+    precision are measured on the 16x16 workload)."""
+    O, mean, eb = last["O"], last["mean"], last["ebar"]
+    rep = gram_rate(O, mean, torch)
     ref_solver = qtx.optimizer.Solver(solver.mode, solver.rshift, solver.ashift, solver.rtol, solver.atol, "fp64")
     x_m = qtx.optimizer.solve_step(O, mean, eb, solver, None, ws_solve)
     x_p = qtx.optimizer.solve_step(O, mean, eb, ref_solver)
@@ -399,11 +407,14 @@ def main():
             traffic = int(1024 * (p["fetch_kib"] * p["fetch_scale"] + p["write_kib"] * p["write_scale"]))
         roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(ach / peak, 4), "traffic": traffic,
+                "hbm_gbps": None if traffic is None else round(traffic / t / 1e9, 3),
+                "hbm_peak_gbps": HBM_PEAK_GBS,
                 "traffic_algorithmic": cfg["ns"] * sweep_steps * 16 + 2 * cfg["ns"] * N + 24 * cfg["ns"],
                 "kernel": "rc_metropolis_kernel", "flops_per_launch": flops,
                 "avg_launch_ms": round(r["metropolis_kernel_ms"], 3)}
     out = {
-        "metric": "VMC steps/s, 16x16 J1J2 ResConv MinSR (mixed vs FP64)" if args.config == "c3" else
+        "metric": "VMC steps/s, 16x16 J1J2 ResConv MinSR (mixed vs FP64); served p50/p99 latency"
+        if args.config == "c3" else
         f"VMC steps/s ({args.config})",
         "value": round(r["steps_per_s"], 5),
         "unit": "VMC steps/s",
@@ -435,6 +446,15 @@ def main():
         out["precision"] = r["precision"]
     if "gram" in r:
         out["gram"] = r["gram"]
+    if cfg["model"] == "resconv":
+        # fused E_loc: one forward per valid connected configuration; reads the
+        # spins and log psi, writes E_loc and the counts (compute-bound)
+        fwd = conv_flops_per_forward(cfg["nblocks"], cfg["channels"], N)
+        t_e = r["eloc_ms"] / 1e3
+        nbytes = cfg["ns"] * (N + 8 + 8 + 4)
+        out["eloc"] = {"kernel": "rc_eloc_kernel", "ms": round(r["eloc_ms"], 2),
+                       "achieved_tflops": round(cfg["ns"] * r["nconn"] * fwd / t_e / 1e12, 1),
+                       "hbm_algorithmic_bytes": nbytes, "hbm_gbps": round(nbytes / t_e / 1e9, 4)}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds, sweep_steps)
     if not args.no_serve and args.config == "c3":
