@@ -180,9 +180,13 @@ def run_mode(cfg, mode, args, torch, qtx):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     energies = []
+    kept = []  # each timed step's (O, mean, Ebar, step) for the per-step Gram-precision check below
+    per_step = mode == "mixed" and cfg["model"] == "resconv" and cfg["L"] == 16
     for i in range(args.steps):
         vmc_step()
         energies.append(opt.energy)
+        if per_step:
+            kept.append((last["O"], last["mean"], last["ebar"], last["step"]))
         print(f"[bench] {mode}: timed step {i + 1}/{args.steps}, E = {opt.energy:.6f}, "
               f"|step| = {last['step_norm']:.3e}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
@@ -195,6 +199,16 @@ def run_mode(cfg, mode, args, torch, qtx):
     out["eloc_ms"] = out["stages"]["eloc"]
     if mode == "mixed" and cfg["model"] == "resconv":
         if cfg["L"] == 16:
+            # per timed step: the split-bf16 solve against the same route with the
+            # FP64 Gram on that step's own O and E_loc (after the timed region)
+            ref_solver = qtx.optimizer.Solver(solver.mode, solver.rshift, solver.ashift, solver.rtol, solver.atol,
+                                              "fp64")
+            errs = []
+            for O_k, m_k, e_k, x_k in kept:
+                x_ref = qtx.optimizer.solve_step(O_k, m_k, e_k, ref_solver)
+                errs.append((torch.linalg.norm(x_k - x_ref) / torch.linalg.norm(x_ref)).item())
+            kept.clear()
+            out["per_step_gram_err"] = errs
             out["gram"] = gram_rate(last["O"], last["mean"], torch)
             out["precision"] = precision_report(cfg, state, H, solver, last, torch, qtx, args.precision_breakdown)
         else:
@@ -444,6 +458,7 @@ def main():
         out["mixed_over_plain"] = round(r["steps_per_s"] / p["steps_per_s"], 3)
     if "precision" in r:
         out["precision"] = r["precision"]
+        out["precision"]["per_step_update_vector_rel_err_gram"] = r.get("per_step_gram_err")
     if "gram" in r:
         out["gram"] = r["gram"]
     if cfg["model"] == "resconv":
