@@ -10,6 +10,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -793,6 +794,7 @@ struct SolveWs {
   double* E;       // n
   double* scalars; // 8
   int32_t* info;   // 4 (+pad)
+  double* dtile;   // 3 x 64 x 64: factored diagonal tiles of the blocked Cholesky, staged
   // partial-fraction soft inverse (EIGH_PINV modes only)
   double* T2;      // 3 x (n x n): T^2 (lower), then the three (T - a_k)^2 + b_k^2
                    // (the first in place of T^2) and their Cholesky factors
@@ -823,6 +825,7 @@ static size_t ws_layout(int64_t ns, int64_t np, int mode, int gram, SolveWs* w, 
   s.E = (double*)take((size_t)n * 8);
   s.scalars = (double*)take(64);
   s.info = (int32_t*)take(3 * kInfoPerFactor * sizeof(int32_t));
+  s.dtile = (double*)take((size_t)3 * 64 * 64 * 8);
   if (mode == QTX_SOLVE_EIGH_PINV) {
     s.T2 = (double*)take((size_t)3 * n * n * 8);
     s.R = (double*)take((size_t)n * 48);
@@ -972,7 +975,8 @@ constexpr int CT = 64;
 // t-th tile below it inside the diagonal block, X = A_t L^{-T}, by forward
 // substitution along the columns in the same pattern.
 __global__ __launch_bounds__(256) void chol_panel_kernel(double* A, int64_t lda, int64_t stride, int64_t d0,
-                                                         int64_t kb, int kp, int32_t* info, int64_t info_stride) {
+                                                         int64_t kb, int kp, int32_t* info, int64_t info_stride,
+                                                         double* __restrict__ dtile) {
   __shared__ double Ls[CT][CT + 1];
   __shared__ double col[2][CT];
   __shared__ double rdg[CT];
@@ -1017,11 +1021,11 @@ __global__ __launch_bounds__(256) void chol_panel_kernel(double* A, int64_t lda,
       int32_t* slot = info + (int64_t)blockIdx.y * info_stride;
       if (*slot == 0) *slot = kp * CT + bad + 1;
     }
+    // staged, not stored into A: the other workgroups of this launch read the
+    // unfactored tile from A, possibly after this one is done
+    double* dt = dtile + (int64_t)blockIdx.y * CT * CT;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int c = c0 + i;
-      if (r < w && c < w && r >= c) Ab[(p0 + r) + (p0 + c) * lda] = a[i];
-    }
+    for (int i = 0; i < 16; ++i) dt[(c0 + i) * CT + r] = a[i];
     return;
   }
 #pragma unroll
@@ -1063,17 +1067,28 @@ __global__ __launch_bounds__(256) void chol_panel_kernel(double* A, int64_t lda,
   }
 }
 
-// A_ij -= L_i L_j^T for the tiles (i, j), kp < j <= i, of the diagonal block
-// (lower triangle only on diagonal tiles); one 64 x 64 tile per workgroup,
-// 4 x 4 outputs per thread
+// Workgroup 0: the panel's factored diagonal tile from its staging copy into
+// A.  Workgroups 1..: A_ij -= L_i L_j^T for the tiles (i, j), kp < j <= i, of
+// the diagonal block (lower triangle only on diagonal tiles); one 64 x 64 tile
+// per workgroup, 4 x 4 outputs per thread
 __global__ __launch_bounds__(256) void chol_update_kernel(double* A, int64_t lda, int64_t stride, int64_t d0,
-                                                          int64_t kb, int kp) {
+                                                          int64_t kb, int kp, const double* __restrict__ dtile) {
   __shared__ double Li[CT][CT + 1];
   __shared__ double Lj[CT][CT + 1];
   double* Ab = A + (int64_t)blockIdx.y * stride;
   const int tid = threadIdx.x;
+  if (blockIdx.x == 0) {  // the panel's factored diagonal tile into A (no tile below reads it)
+    const int64_t p0 = d0 + (int64_t)kp * CT;
+    const int w = (int)(kb - (int64_t)kp * CT < CT ? kb - (int64_t)kp * CT : CT);
+    const double* dt = dtile + (int64_t)blockIdx.y * CT * CT;
+    for (int e = tid; e < CT * CT; e += 256) {
+      const int r = e % CT, c = e / CT;
+      if (r < w && c < w && r >= c) Ab[(p0 + r) + (p0 + c) * lda] = dt[c * CT + r];
+    }
+    return;
+  }
   int ti, tj;
-  tile_of(blockIdx.x, ti, tj);
+  tile_of(blockIdx.x - 1, ti, tj);
   const int i = kp + 1 + ti, j = kp + 1 + tj;
   const int64_t p0 = d0 + (int64_t)kp * CT;
   const int w = (int)(kb - (int64_t)kp * CT < CT ? kb - (int64_t)kp * CT : CT);
@@ -1113,7 +1128,7 @@ __global__ __launch_bounds__(256) void chol_update_kernel(double* A, int64_t lda
 }
 
 static int blocked_potrf(rocblas_handle h, hipStream_t st, int64_t n, double* A, int64_t stride, int batch,
-                         int32_t* info, int64_t info_stride) {
+                         int32_t* info, int64_t info_stride, double* dtile) {
   const double one = 1.0, minus_one = -1.0;
   const rocblas_int ld = (rocblas_int)n;
   int b = 0;
@@ -1122,11 +1137,10 @@ static int blocked_potrf(rocblas_handle h, hipStream_t st, int64_t n, double* A,
     const int nt = (int)((kb + CT - 1) / CT);
     for (int kp = 0; kp < nt; ++kp) {
       hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)(nt - kp), (unsigned)batch), dim3(256), 0, st, A,
-                         (int64_t)n, stride, k, kb, kp, info + b, info_stride);
+                         (int64_t)n, stride, k, kb, kp, info + b, info_stride, dtile);
       const int m = nt - kp - 1;
-      if (m > 0)
-        hipLaunchKernelGGL(chol_update_kernel, dim3((unsigned)(m * (m + 1) / 2), (unsigned)batch), dim3(256), 0, st,
-                           A, (int64_t)n, stride, k, kb, kp);
+      hipLaunchKernelGGL(chol_update_kernel, dim3((unsigned)(1 + m * (m + 1) / 2), (unsigned)batch), dim3(256), 0,
+                         st, A, (int64_t)n, stride, k, kb, kp, (const double*)dtile);
     }
     QTX_LAUNCH_CHECK();
     const int64_t m = n - k - kb;
@@ -1162,13 +1176,13 @@ static int blocked_potrf(rocblas_handle h, hipStream_t st, int64_t n, double* A,
 //            for the partial fractions; dsyevd where that route does not apply.
 //   phase 3: read the factorisation status (sync); dsyevd fallback; the update
 //            vector x = Obar^T y.
-// Only the MFMA Gram and the Lanczos matvecs of different tasks overlap: a
-// task's rocSOLVER factorisations (potrf/potrs/syrk in phase 1 for
-// CHOL_SHIFT, phase 2 for the partial fractions) are waited for before the
-// next task's are issued.  Run concurrently on different streams and
-// handles, those factorisations failed (potrf info != 0) in runs whose
-// every single solve succeeded; one after another they never failed and the
-// stream of solves was also faster.
+// All phases of different tasks overlap on their streams (the factorisations
+// too, unless QTX_SERIAL_FACTOR=1).  A panel kernel of the blocked Cholesky
+// once stored its factored diagonal tile into the matrix while other
+// workgroups of the same launch could still be reading the unfactored tile;
+// delayed workgroups — common when other tasks occupy the GPU — then factored
+// garbage (non-positive pivots in 249 of 640 overlapped solves); the tile is
+// now staged and stored by the next launch.
 struct Task {
   qtx_minsr_task* io = nullptr;
   hipStream_t st = nullptr;
@@ -1231,7 +1245,7 @@ static int phase1(Task& t, rocblas_handle h) {
     QTX_LAUNCH_CHECK();
     hipLaunchKernelGGL(fill_i32_kernel, dim3(nblk(3 * kInfoPerFactor, 256)), dim3(256), 0, t.st, t.w.info,
                        (int64_t)(3 * kInfoPerFactor), 0);
-    rc = blocked_potrf(h, t.st, t.n, t.w.T, 0, 1, t.w.info, kInfoPerFactor);
+    rc = blocked_potrf(h, t.st, t.n, t.w.T, 0, 1, t.w.info, kInfoPerFactor, t.w.dtile);
     if (rc) return rc;
     return rb(rocsolver_dpotrs(h, rocblas_fill_lower, ni, 1, t.w.T, ni, t.w.y, ni), "rocsolver_dpotrs");
   }
@@ -1326,7 +1340,7 @@ static int phase2(Task& t, rocblas_handle h) {
                            t.w.T2 + k * nn);
         QTX_LAUNCH_CHECK();
       }
-      rc = blocked_potrf(h, t.st, t.n, t.w.T2, nn, 3, t.w.info, kInfoPerFactor);
+      rc = blocked_potrf(h, t.st, t.n, t.w.T2, nn, 3, t.w.info, kInfoPerFactor, t.w.dtile);
       if (rc) return rc;
       // iterative refinement of the three complex systems (T - p_k) z = y,
       // the inner solves batched over the three factors, the products with T
@@ -1423,6 +1437,16 @@ static int check_task(const qtx_minsr_task& io) {
   return QTX_OK;
 }
 
+// QTX_SERIAL_FACTOR=1 runs the factorisations of different tasks one after
+// another (default: they overlap on their streams)
+static bool serial_factorisations() {
+  static const bool v = [] {
+    const char* e = getenv("QTX_SERIAL_FACTOR");
+    return e && e[0] == '1';
+  }();
+  return v;
+}
+
 static int solve_many(qtx_minsr_task* tasks, int ntasks) {
   std::vector<Task> ts((size_t)ntasks);
   int first_rc = QTX_OK;
@@ -1465,7 +1489,8 @@ static int solve_many(qtx_minsr_task* tasks, int ntasks) {
         else rc = f32 ? phase3<float>(t, h) : phase3<double>(t, h);
         // factorisations of one task finish before the next task's start
         const bool factored = (phase == 1 && t.io->mode == QTX_SOLVE_CHOL_SHIFT) || (phase == 2 && t.pf);
-        if (rc == QTX_OK && factored && ntasks > 1) rc = hip_check(hipStreamSynchronize(t.st), "hipStreamSynchronize");
+        if (rc == QTX_OK && factored && ntasks > 1 && serial_factorisations())
+          rc = hip_check(hipStreamSynchronize(t.st), "hipStreamSynchronize");
         rocblas_set_pointer_mode(h, pm);
       }
       if (rc) note(t, rc);

@@ -243,7 +243,7 @@ class JobRun:
 
     def result(self) -> dict:
         return {"job_id": self.spec.job_id, "energy": self.energy.cpu().numpy(),
-                "theta": self.model.theta.detach().cpu().numpy()}
+                "theta": self.model.theta.detach().cpu().numpy(), "solve_paths": list(self.solve_paths)}
 
 
 def admit(pending: List[JobSpec], now: float, used: int, budget: int, cost) -> List[JobSpec]:

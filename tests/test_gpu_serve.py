@@ -131,3 +131,49 @@ def test_solve_many_equals_single_solves(rng):
     for j in jobs:
         err = (torch.linalg.norm(j["out"] - j["ref"]) / torch.linalg.norm(j["ref"])).item()
         assert err < 1e-12, err
+
+
+def test_solve_many_overlapped_factorisations_stay_exact(rng):
+    """Twelve partial-fraction solves (n = 2048, a load where GPU time is
+    shared) overlapping on their streams, three times over: no factorisation
+    may fail, and each result equals the task's own solve alone.  A panel
+    kernel that stored its factored diagonal tile while late workgroups of the
+    same launch still read it failed here in about 4 of 10 solves."""
+    from quantax_amd import _lib
+    dev = torch.device("cuda")
+    ns, np_ = 2048, 5000
+    gram, mode = _lib.QTX_GRAM_BF16X3, _lib.QTX_SOLVE_EIGH_PINV
+    nws = int(_lib.lib().qtx_minsr_ws_bytes(ns, np_, mode, gram))
+    jobs = []
+    for k in range(12):
+        O = torch.from_numpy((rng.standard_normal((ns, np_)) * 0.3 + 0.1).astype(np.float32)).to(dev)
+        O[:, : np_ // 2] *= 0.05
+        E = torch.from_numpy(rng.standard_normal(ns) - 2.0).to(dev)
+        mean = torch.empty(np_, dtype=torch.float64, device=dev)
+        _lib.call("qtx_colmean", _lib.stream(), _lib.QTX_F32, O.data_ptr(), ns, np_, mean.data_ptr())
+        eb = torch.empty(ns, dtype=torch.float64, device=dev)
+        o2 = torch.empty(2, dtype=torch.float64, device=dev)
+        _lib.call("qtx_ebar", _lib.stream(), E.data_ptr(), None, ns, eb.data_ptr(), o2.data_ptr())
+        ws = torch.empty(nws, dtype=torch.uint8, device=dev)
+        ref = torch.empty(np_, dtype=torch.float64, device=dev)
+        _lib.call("qtx_minsr_solve", _lib.stream(), _lib.QTX_F32, O.data_ptr(), mean.data_ptr(), None, eb.data_ptr(),
+                  ns, np_, mode, gram, 0.0, 0.0, 1e-4, 0.0, ref.data_ptr(), ws.data_ptr(), nws)
+        jobs.append(dict(O=O, mean=mean, eb=eb, ws=ws, ref=ref, out=torch.empty_like(ref),
+                         stream=torch.cuda.Stream()))
+    torch.cuda.synchronize()
+    for rep in range(3):
+        tasks = (_lib.QtxMinsrTask * len(jobs))()
+        for t, j in zip(tasks, jobs):
+            t.stream = j["stream"].cuda_stream
+            t.o_dtype, t.mode, t.gram = _lib.QTX_F32, mode, gram
+            t.O, t.mean, t.r, t.ebar = j["O"].data_ptr(), j["mean"].data_ptr(), None, j["eb"].data_ptr()
+            t.ns, t.np = ns, np_
+            t.rshift, t.ashift, t.rtol, t.atol = 0.0, 0.0, 1e-4, 0.0
+            t.step, t.ws, t.ws_bytes = j["out"].data_ptr(), j["ws"].data_ptr(), j["ws"].numel()
+        assert _lib.lib().qtx_minsr_solve_many(tasks, len(jobs)) == 0
+        torch.cuda.synchronize()
+        assert all(t.status == 0 and t.path == 1 and t.fallback == 0 for t in tasks), \
+            [(t.path, t.fallback) for t in tasks]
+        for j in jobs:
+            err = (torch.linalg.norm(j["out"] - j["ref"]) / torch.linalg.norm(j["ref"])).item()
+            assert err < 1e-9, err
