@@ -255,7 +255,8 @@ int qtx_scale_columns(void* stream, int o_dtype, void* O, double* mean, int64_t 
  *                  iterative refinement against T; lambda_max by Lanczos.  Smaller
  *                  cutoffs, or a failed factorisation, use rocSOLVER dsyevd.
  * mode EIGH_PINV_DIRECT: the same g(T) b through dsyevd always.
- * gram: QTX_GRAM_* (the ns >= np branch always uses FP64 products).
+ * gram: QTX_GRAM_* (the ns >= np branch always uses FP64 products: Obar formed
+ *       in FP64 by row blocks of at most 256 MiB, S accumulated by rocBLAS DGEMM).
  * ws: qtx_minsr_ws_bytes(...) bytes of device memory. */
 size_t qtx_minsr_ws_bytes(int64_t ns, int64_t np, int mode, int gram);
 /* The Gram matrix alone, FP64 row-major: T = Obar Obar^T (ns x ns) when

@@ -419,6 +419,18 @@ __global__ __launch_bounds__(256) void gram_tn_kernel(const TI* __restrict__ O, 
   if (k < np && l < np) S[k * np + l] = acc;
 }
 
+// out[s, k] = (O[s, k] - mean[k]) sqrt(r_s / ns) in FP64 (grid-stride)
+template <typename TI>
+__global__ void form_obar_kernel(const TI* __restrict__ O, const double* __restrict__ mean,
+                                 const double* __restrict__ r, int64_t ns, int64_t np, double inv_ns,
+                                 double* __restrict__ out) {
+  const int64_t total = ns * np;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = e / np, k = e - s * np;
+    out[e] = ((double)O[e] - mean[k]) * sqrt((r ? r[s] : 1.0) * inv_ns);
+  }
+}
+
 // x_k = sum_s (O_sk - mean_k) sqrt(r_s / ns) y_s  (FP64 accumulation)
 template <typename TI>
 __global__ __launch_bounds__(256) void obar_t_y_kernel(const TI* __restrict__ O, const double* __restrict__ mean,
@@ -786,6 +798,11 @@ constexpr int kLanczosMax = 512;  // Lanczos steps for lambda_max before giving 
 constexpr int64_t kCholBlock = QTX_CHOL_BLOCK;  // diagonal block of the blocked Cholesky
 constexpr int kInfoPerFactor = (int)((131072 + kCholBlock - 1) / kCholBlock);  // status slots: n <= 131072
 
+// Rows of FP64 Obar formed per DGEMM chunk in the SR branch (<= 256 MiB)
+static int64_t obar_rows(int64_t ns, int64_t np) {
+  return std::max<int64_t>(1, std::min<int64_t>(ns, (int64_t(1) << 28) / (8 * std::max<int64_t>(np, 1))));
+}
+
 struct SolveWs {
   double* T;       // n x n
   double* y;       // max(ns, np)
@@ -806,6 +823,7 @@ struct SolveWs {
   double* w;       // n: Lanczos work vector
   double* ab;      // alpha[kLanczosMax], beta[kLanczosMax + 1]
   void* split;     // bf16 hi / lo planes of Obar (split-bf16 Gram only)
+  double* obar;    // ns x np FP64 Obar (SR branch, ns >= np: S = Obar^T Obar by DGEMM)
 };
 
 static size_t ws_layout(int64_t ns, int64_t np, int mode, int gram, SolveWs* w, char* base) {
@@ -837,6 +855,7 @@ static size_t ws_layout(int64_t ns, int64_t np, int mode, int gram, SolveWs* w, 
     s.ab = (double*)take((size_t)(2 * kLanczosMax + 1) * 8);
   }
   if (gram == QTX_GRAM_BF16X3 && ns < np) s.split = take(g3_bytes(ns, np));
+  if (ns >= np) s.obar = (double*)take((size_t)obar_rows(ns, np) * np * 8);
   if (w) *w = s;
   return off;
 }
@@ -1231,10 +1250,23 @@ static int phase1(Task& t, rocblas_handle h) {
     if (rc) return rc;
     hipLaunchKernelGGL(copy_f64_kernel, dim3(nblk(io.ns, 256)), dim3(256), 0, t.st, io.ebar, t.w.y, io.ns);
   } else {
-    dim3 g(nblk(io.np, 16), nblk(io.np, 16));
-    hipLaunchKernelGGL(gram_tn_kernel<TI>, g, dim3(256), 0, t.st, O, io.mean, io.r, io.ns, io.np,
-                       1.0 / (double)io.ns, t.w.T);
-    QTX_LAUNCH_CHECK();
+    // S = Obar^T Obar: Obar formed in FP64 a row block at a time, each block
+    // accumulated by one rocBLAS DGEMM (a row-major rows x np block is the
+    // column-major np x rows matrix B, S += B B^T)
+    const int64_t rows = obar_rows(io.ns, io.np);
+    const double one = 1.0, zero = 0.0;
+    for (int64_t s0 = 0; s0 < io.ns; s0 += rows) {
+      const int64_t nr = std::min(rows, io.ns - s0), tot = nr * io.np;
+      hipLaunchKernelGGL(form_obar_kernel<TI>, dim3((unsigned)std::min<int64_t>((tot + 255) / 256, kGridCap)),
+                         dim3(256), 0, t.st, O + s0 * io.np, io.mean, io.r ? io.r + s0 : nullptr, nr, io.np,
+                         1.0 / (double)io.ns, t.w.obar);
+      QTX_LAUNCH_CHECK();
+      rc = rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)io.np,
+                            (rocblas_int)io.np, (rocblas_int)nr, &one, t.w.obar, (rocblas_int)io.np, t.w.obar,
+                            (rocblas_int)io.np, s0 ? &one : &zero, t.w.T, (rocblas_int)io.np),
+              "rocblas_dgemm");
+      if (rc) return rc;
+    }
     rc = launch_obar_t_y<TI>(t.st, O, io.mean, io.r, io.ebar, io.ns, io.np, t.w.y);  // F = Obar^T ebar
     if (rc) return rc;
   }
