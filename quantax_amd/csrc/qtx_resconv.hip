@@ -27,6 +27,7 @@ namespace qtx {
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr int RC_WAVES = 4;
 constexpr int RC_THREADS = RC_WAVES * WAVE;
@@ -139,10 +140,29 @@ __global__ void rc_pack_frag_kernel(const float* __restrict__ theta, int nb, cha
     reinterpret_cast<float*>(dst)[((int64_t)ct * KKS + ks) * WAVE + lane] = v;
   } else {
     int tap, c0;
+    __bf16* o = reinterpret_cast<__bf16*>(dst) + (((int64_t)ct * KKS + ks) * WAVE + lane) * 16;
+    if (WIN && (ks % 2) == 1) {
+      // tap (dy, 2) holds 16 of a K-step's 32 rows, so its three split
+      // products share instructions: slot 0 of K-step 2 dy + 1 is
+      // [hi | lo] of tap (dy, 2) (against [X hi; X hi]); slot 1 carries the
+      // hi-times-X-lo halves: [hi (0, 2) | hi (1, 2)] in K-step 1,
+      // [hi (2, 2) | 0] in K-step 3, zeros in K-step 5
+      const int dy = ks / 2, up = g >> 1;
+      c0 = 8 * (g & 1);
+      for (int q = 0; q < 8; ++q) {
+        const float v = W(c0 + q, 3 * dy + 2);
+        const __bf16 hi = (__bf16)v;
+        o[q] = up ? (__bf16)(v - (float)hi) : hi;
+        float v1 = 0.0f;
+        if (dy == 0) v1 = W(c0 + q, up ? 5 : 2);
+        else if (dy == 1 && !up) v1 = W(c0 + q, 8);
+        o[8 + q] = (__bf16)v1;
+      }
+      return;
+    }
     if (WIN) {
-      // ks = 2 dy + pr; pr 0: taps (dy, 0), (dy, 1); pr 1: tap (dy, 2) and a zero pad
-      int dy = ks / 2, pr = ks % 2;
-      tap = (pr == 0) ? 3 * dy + (g >> 1) : ((g >> 1) == 0 ? 3 * dy + 2 : 9);
+      // ks = 2 dy: taps (dy, 0), (dy, 1)
+      tap = 3 * (ks / 2) + (g >> 1);
       c0 = 8 * (g & 1);
     } else if (C == 16) {
       tap = 2 * ks + (g >> 1);
@@ -151,7 +171,6 @@ __global__ void rc_pack_frag_kernel(const float* __restrict__ theta, int nb, cha
       tap = ks;
       c0 = 8 * g;
     }
-    __bf16* o = reinterpret_cast<__bf16*>(dst) + (((int64_t)ct * KKS + ks) * WAVE + lane) * 16;
     for (int q = 0; q < 8; ++q) {
       float v = (tap < 9) ? W(c0 + q, tap) : 0.0f;
       __bf16 hi = (__bf16)v;
@@ -869,17 +888,24 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
                 acc = __builtin_amdgcn_mfma_f32_16x16x4f32(wf[(3 * dy + dx) * 4 + jp], R.f[dx][jp], acc, 0, 0, 0);
           }
         } else {
+          // taps (dy, 0), (dy, 1): three split products per K-step; tap
+          // (dy, 2): [hi | lo] x [X hi; X hi] per dy, then the hi x X lo
+          // halves two dy to an instruction (14 MFMAs per row instead of 18)
 #pragma unroll
           for (int dy = 0; dy < 3; ++dy) {
             const Frag& R = (dy == 0) ? Fm : (dy == 1 ? F0 : Fp);
-#pragma unroll
-            for (int pr = 0; pr < 2; ++pr) {
-              const int ks = 2 * dy + pr;
-              acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[ks], R.h[pr], acc, 0, 0, 0);
-              acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[ks], R.l[pr], acc, 0, 0, 0);
-              acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[ks], R.h[pr], acc, 0, 0, 0);
-            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[2 * dy], R.h[0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[2 * dy], R.l[0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[2 * dy], R.h[0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[2 * dy + 1], R.h[1], acc, 0, 0, 0);
           }
+          const v4i m4 = __builtin_bit_cast(v4i, Fm.l[1]), z4 = __builtin_bit_cast(v4i, F0.l[1]);
+          v4i s4;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) s4[q] = (lane < 32) ? m4[q] : z4[q];
+          const v8bf lo01 = __builtin_bit_cast(v8bf, s4);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[1], lo01, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[3], Fp.l[1], acc, 0, 0, 0);
         }
       }
       if (i > 0) epi(i - 1, accp);
