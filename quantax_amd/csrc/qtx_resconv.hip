@@ -216,6 +216,20 @@ __device__ __forceinline__ float gelu_grad_f(float x, int exact) {
   return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * 0.79788456080286536f * (1.0f + 0.134145f * x * x);
 }
 
+// v = hi + lo + O(2^-16 |v|): hi = bf16(v) (round to nearest even), lo =
+// bf16(v - hi).  hi is converted once per pair and widened back to FP32 by
+// bit moves (bf16 is the top half of an FP32 word).
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split_bf16x4(v4f v, v4bf& hi, v4bf& lo) {
+  const v2bf h01 = {(__bf16)v[0], (__bf16)v[1]}, h23 = {(__bf16)v[2], (__bf16)v[3]};
+  const uint32_t u01 = __builtin_bit_cast(uint32_t, h01), u23 = __builtin_bit_cast(uint32_t, h23);
+  const float f0 = __builtin_bit_cast(float, u01 << 16), f1 = __builtin_bit_cast(float, u01 & 0xffff0000u);
+  const float f2 = __builtin_bit_cast(float, u23 << 16), f3 = __builtin_bit_cast(float, u23 & 0xffff0000u);
+  const v2bf l01 = {(__bf16)(v[0] - f0), (__bf16)(v[1] - f1)}, l23 = {(__bf16)(v[2] - f2), (__bf16)(v[3] - f3)};
+  hi = v4bf{h01[0], h01[1], h23[0], h23[1]};
+  lo = v4bf{l01[0], l01[1], l23[0], l23[1]};
+}
+
 // ---------------------------------------------------------------- forward core
 template <int C>
 __device__ __forceinline__ int swz_chunk(int p, int j) {
@@ -244,12 +258,7 @@ __device__ __forceinline__ void act_store4(char* act_cfg, int p, int c4, v4f v) 
     *reinterpret_cast<v4f*>(act_cfg + ((int64_t)p * CH + swz_chunk<C>(p, j)) * 16) = v;
   } else {
     v4bf hi, lo;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      __bf16 h = (__bf16)v[q];
-      hi[q] = h;
-      lo[q] = (__bf16)(v[q] - (float)h);
-    }
+    split_bf16x4(v, hi, lo);
     int j = c4 / 8, half = (c4 % 8) / 4;
     char* base = act_cfg + (int64_t)p * CH * 16;
     *reinterpret_cast<v4bf*>(base + swz_chunk<C>(p, j) * 16 + half * 8) = hi;
@@ -535,12 +544,7 @@ __device__ __attribute__((always_inline)) inline void rc_forward_rows(const RcGe
       *reinterpret_cast<v4f*>(row_base + st_f32) = v;
     } else {
       v4bf hi, lo;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        __bf16 h = (__bf16)v[q];
-        hi[q] = h;
-        lo[q] = (__bf16)(v[q] - (float)h);
-      }
+      split_bf16x4(v, hi, lo);
       *reinterpret_cast<v4bf*>(row_base + st_hi) = hi;
       *reinterpret_cast<v4bf*>(row_base + st_lo) = lo;
     }
@@ -774,12 +778,7 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
       *reinterpret_cast<v4f*>(row_base + st_f32) = v;
     } else {
       v4bf hi, lo;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        __bf16 h = (__bf16)v[q];
-        hi[q] = h;
-        lo[q] = (__bf16)(v[q] - (float)h);
-      }
+      split_bf16x4(v, hi, lo);
       *reinterpret_cast<v4bf*>(row_base + st_hi) = hi;
       *reinterpret_cast<v4bf*>(row_base + st_lo) = lo;
     }
@@ -957,36 +956,38 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
     const int l = 2 * nb - 1;
     const char* in = act + cur * BUFB + b * CFGB;
     const float scale = rsqrtf((float)(nb + 1));
-    float m_w = -INFINITY, s_w = 0.0f;
+    // log-sum-exp of the head: each lane keeps a running (max, sum) over its
+    // own values of the wave's rows; one wave reduction at the end
+    float m_l = -INFINITY, s_l = 0.0f;
     run_layer(l, in, [&](int i, v4f acc) {
-      float hv[4], m = -INFINITY;
+      float hv[4];
       const v4f r = resv[i * WAVE];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        hv[q] = (acc[q] + r[q]) * scale;
-        m = fmaxf(m, hv[q]);
-      }
-      m = wave_max(m);
-      float sacc = 0.0f;
+      for (int q = 0; q < 4; ++q) hv[q] = (acc[q] + r[q]) * scale;
+      const float mn = fmaxf(m_l, fmaxf(fmaxf(hv[0], hv[1]), fmaxf(hv[2], hv[3])));
+      float sacc = s_l * __expf(m_l - mn);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) sacc += __expf(hv[q] - m);
-      sacc = wave_sum(sacc);
-      float mm = fmaxf(m_w, m);
-      s_w = s_w * __expf(m_w - mm) + sacc * __expf(m - mm);
-      m_w = mm;
+      for (int q = 0; q < 4; ++q) sacc += __expf(hv[q] - mn);
+      s_l = sacc;
+      m_l = mn;
     });
+    const float m_w = wave_max(m_l);
+    const float s_w = wave_sum(s_l * __expf(m_l - m_w));
     if (lane == 0) {
       red[wv * 2] = m_w;
       red[wv * 2 + 1] = s_w;
     }
     __syncthreads();
     if (threadIdx.x < B) {
-      int bb = threadIdx.x;
+      // merge the four waves' (max, sum) in FP32 (the sums are FP32 already);
+      // FP64 exp / log here would hold every wave at the barrier below
+      constexpr double kLogCN = 8.317766166719343;  // log(16 channels * 256 sites)
+      const int bb = threadIdx.x;
       float m = -INFINITY;
       for (int w = 4 * bb; w < 4 * bb + 4; ++w) m = fmaxf(m, red[w * 2]);
-      double ssum = 0.0;
-      for (int w = 4 * bb; w < 4 * bb + 4; ++w) ssum += (double)red[w * 2 + 1] * exp((double)red[w * 2] - (double)m);
-      lp_out[bb] = (double)m + log(ssum) - log((double)C * 256.0);
+      float ssum = 0.0f;
+      for (int w = 4 * bb; w < 4 * bb + 4; ++w) ssum += red[w * 2 + 1] * __expf(red[w * 2] - m);
+      lp_out[bb] = (double)m + (double)logf(ssum) - kLogCN;
     }
     __syncthreads();
   }
@@ -1067,6 +1068,9 @@ __global__ __launch_bounds__(W * 64, ((W == 8 || ROWS == 2) ? 2 : 1)) void rc_me
     if (lane == 0) nhop[b] = cnt;
   }
   __syncthreads();
+  // with at least two waves per chain, the FP64 acceptance threshold of
+  // chain b is computed by wave B + b while wave b draws the move
+  constexpr bool kThrWave = (2 * B <= W);
   for (int t = 0; t < nsteps; ++t) {
     // proposals: one wave per chain
     for (int b = wv; b < B; b += W) {
@@ -1077,7 +1081,7 @@ __global__ __launch_bounds__(W * 64, ((W == 8 || ROWS == 2) ? 2 : 1)) void rc_me
       if (c < ns) {
         uint4 w = reinterpret_cast<const uint4*>(words)[(int64_t)t * ns + c];
         propose_move(kind, cur + b * N, N, nbr, width, hopping, nhop[b], w.x, w.y, lane, fa, fb, upd);
-        thr = log1p(-words_to_uniform(w.z, w.w));
+        if constexpr (!kThrWave) thr = log1p(-words_to_uniform(w.z, w.w));
       }
       for (int i = lane; i < N; i += WAVE) {
         int8_t v = cur[b * N + i];
@@ -1088,7 +1092,19 @@ __global__ __launch_bounds__(W * 64, ((W == 8 || ROWS == 2) ? 2 : 1)) void rc_me
         mv_a[b] = fa;
         mv_b[b] = fb;
         mv_upd[b] = upd ? 1 : 0;
-        mv_thr[b] = thr;
+        if constexpr (!kThrWave) mv_thr[b] = thr;
+      }
+    }
+    if constexpr (kThrWave) {
+      if (wv >= B && wv < 2 * B) {
+        const int b = wv - B;
+        const int64_t c = c0 + b;
+        double thr = 0.0;
+        if (c < ns) {
+          uint4 w = reinterpret_cast<const uint4*>(words)[(int64_t)t * ns + c];
+          thr = log1p(-words_to_uniform(w.z, w.w));
+        }
+        if (lane == 0) mv_thr[b] = thr;
       }
     }
     __syncthreads();
