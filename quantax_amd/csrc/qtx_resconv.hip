@@ -1182,21 +1182,51 @@ __global__ __launch_bounds__(W * 64, ((W == 8 || ROWS == 2) ? 2 : 1)) void rc_el
   __syncthreads();
   const int nvalid = nvalid_s;
   const double lp0 = logabs[n];
+  // prop[b] holds s0 with candidate start + b applied: thread b < B undoes
+  // the previous candidate's (at most two) flips and applies the next one's,
+  // whose sites were fetched before the forward
+  for (int i = threadIdx.x; i < B * N; i += (W * WAVE)) prop[i] = s0[i % N];
+  int fa = -1, fb = -1, na = -1, nb2 = -1;
+  if (threadIdx.x < B && (int)threadIdx.x < nvalid) {
+    const int k = vlist[threadIdx.x];
+    na = H.off_a[k];
+    nb2 = H.off_b[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < B) {
+    int8_t* pb = prop + threadIdx.x * N;
+    if (na >= 0) pb[na] = -s0[na];
+    if (nb2 >= 0) pb[nb2] = -s0[nb2];
+    fa = na;
+    fb = nb2;
+  }
+  __syncthreads();
   double e = 0.0;  // accumulated by thread 0 in candidate order
   for (int start = 0; start < nvalid; start += B) {
-    for (int i = threadIdx.x; i < B * N; i += (W * WAVE)) {
-      int b = i / N, j = i % N;
-      int8_t v = s0[j];
-      if (start + b < nvalid) {
-        int k = vlist[start + b];
-        if (j == H.off_a[k] || j == H.off_b[k]) v = -v;
-      }
-      prop[i] = v;
+    na = nb2 = -1;
+    if (threadIdx.x < B && start + B + (int)threadIdx.x < nvalid) {
+      const int k = vlist[start + B + threadIdx.x];
+      na = H.off_a[k];
+      nb2 = H.off_b[k];
     }
-    __syncthreads();
     RC_FWD(prop, lp);
     if (threadIdx.x == 0) {
-      for (int b = 0; b < B && start + b < nvalid; ++b) e += H.off_c[vlist[start + b]] * exp(lp[b] - lp0);
+      for (int b = 0; b < B && start + b < nvalid; ++b) {
+        // |log ratio| < 80: FP32 exp (relative error ~1e-7, below the
+        // network's own); beyond, FP64 (FP32 would overflow)
+        const double d = lp[b] - lp0;
+        const double r = (fabs(d) < 80.0) ? (double)__expf((float)d) : exp(d);
+        e += H.off_c[vlist[start + b]] * r;
+      }
+    }
+    if (threadIdx.x < B) {
+      int8_t* pb = prop + threadIdx.x * N;
+      if (fa >= 0) pb[fa] = s0[fa];
+      if (fb >= 0) pb[fb] = s0[fb];
+      if (na >= 0) pb[na] = -s0[na];
+      if (nb2 >= 0) pb[nb2] = -s0[nb2];
+      fa = na;
+      fb = nb2;
     }
     __syncthreads();
   }
