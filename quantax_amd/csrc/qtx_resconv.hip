@@ -716,8 +716,9 @@ __device__ __attribute__((always_inline)) inline void rc_forward_rows(const RcGe
 // configuration w / 4 and walks them in order keeping the B fragments of
 // input rows y - 1, y, y + 1 in registers: every fragment depends only on its
 // input row (split-bf16 K-steps 2 dy + pr: pr 0 = taps (dy, 0) + (dy, 1),
-// pr 1 = tap (dy, 2) + a zero pad; FP32 K-steps = taps), so each row is read
-// from LDS once per wave and layer instead of three times.
+// pr 1 = tap (dy, 2) in both K halves, whose split products over the three dy
+// take five instructions per output row, see the packing; FP32 K-steps = taps), so each
+// row is read from LDS once per wave and layer instead of three times.
 //
 // Issue structure: the GELU form and the conv kind (conv1 / conv2 with
 // residual) are compile-time inside a layer (one branch per layer), and the
