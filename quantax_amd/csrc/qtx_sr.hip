@@ -1294,7 +1294,11 @@ static int phase1(Task& t, rocblas_handle h) {
   // recurrence; the largest Ritz value is checked every 32 steps)
   hipLaunchKernelGGL(lanczos_init_kernel, dim3(1), dim3(1024), 0, t.st, t.n, t.w.V, t.w.ab + kLanczosMax);
   t.mcap = (int)(t.n < kLanczosMax ? t.n : kLanczosMax);
-  return lanczos_enqueue(t, h, t.mcap < 48 ? t.mcap : 48);
+  // first convergence check after 32 steps when one product streams more
+  // than 512 MB of T (n > 8192: 16 more products then cost far more than the
+  // host round trip of a check), after 48 otherwise
+  const int first = t.n > 8192 ? 32 : 48;
+  return lanczos_enqueue(t, h, t.mcap < first ? t.mcap : first);
 }
 
 static int phase2(Task& t, rocblas_handle h) {
