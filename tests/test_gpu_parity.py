@@ -462,6 +462,24 @@ def test_resconv_rows_eloc_j1j2_16x16(precision, tol, rng):
     assert max_rel(e.cpu().numpy(), ref) < tol
 
 
+@pytest.mark.parametrize("L,precision,tol", [(4, "fp32", 1e-5), (4, "bf16x3", 1e-4), (16, "fp32", 1e-5),
+                                              (16, "bf16x3", 1e-4)])
+def test_resconv_eloc_ising_single_flips(L, precision, tol, rng):
+    """Transverse-field Ising: every off-diagonal term flips one site, the
+    case where the E_loc kernels apply and undo a single flip in place (4 x 4:
+    tile kernel, 16 x 16: window kernel); every site is a connected
+    configuration."""
+    lat, model, spec, theta = _resconv_rect(L, L, 2, 16, precision)
+    H = qtx.operator.Ising(h=0.7, J=1.3)
+    Ho = oop.ising([L, L], h=0.7, J=1.3)
+    s = rand_spins(rng, 3 if L == 16 else 16, L * L)
+    la = model.logpsi(to_dev(s))
+    e, nconn = model.eloc(H, to_dev(s), la)
+    ref = oop.oloc(s, lambda x: om.resconv_forward(spec, theta, x), Ho)
+    assert max_rel(e.cpu().numpy(), ref) < tol
+    assert (nconn.cpu().numpy() == L * L).all()
+
+
 @pytest.mark.parametrize("precision,tie,lptol", [("fp32", 1e-4, 1e-5), ("bf16x3", 1e-3, 1e-4)])
 def test_resconv_rows_metropolis_16x16(precision, tie, lptol, rng):
     lat, model, spec, theta = _resconv_rect(16, 16, 2, 16, precision, nup=128)
