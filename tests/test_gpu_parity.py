@@ -480,6 +480,40 @@ def test_resconv_eloc_ising_single_flips(L, precision, tol, rng):
     assert (nconn.cpu().numpy() == L * L).all()
 
 
+def test_resconv_eloc_large_log_ratios(rng):
+    """Connected configurations whose |log psi(s') - log psi(s)| passes 80
+    take the FP64 exp in the E_loc kernel (FP32 overflows near 88).  The last
+    convolution's weights are scaled until the oracle sees such ratios (and
+    none beyond FP64's range)."""
+    lat, model, spec, theta0 = _resconv_rect(4, 4, 2, 16, "fp32")
+    H = qtx.operator.Heisenberg(J=[1, 0.5], n_neighbor=[1, 2], msr=True)
+    Ho = oop.heisenberg([4, 4], [1, 0.5], [1, 2], True)
+    s = rand_spins(rng, 16, 16, nup=8)
+    cin, cout, _ = spec.layers[-1]
+    w0, _ = spec.offsets[-1]
+    conn = [oop.get_conn(sc, Hc)[:2] for sc, Hc in oop.apply_off_diag(s, Ho).values()]
+    for scale in (50.0, 100.0, 200.0, 400.0, 800.0):
+        th = theta0.copy()
+        th[w0: w0 + cout * cin * 9] *= scale
+        th = th.astype(np.float32).astype(np.float64)
+        f = lambda x, th=th: om.resconv_forward(spec, th, x)
+        la0 = f(s)[1]
+        d = np.concatenate([np.abs(f(sconn)[1] - la0[seg]) for seg, sconn in conn])
+        if d.max() > 80.0:
+            break
+    assert 80.0 < d.max() < 600.0, d.max()
+    model.set_params(th.astype(np.float32))
+    la = model.logpsi(to_dev(s))
+    e, _ = model.eloc(H, to_dev(s), la)
+    ref = oop.oloc(s, f, Ho)
+    assert np.all(np.isfinite(e.cpu().numpy()))
+    # log psi runs to ~200 here and the scaled layer's pre-activations are as
+    # large: FP32 rounding through the network leaves a few tens of ulps of
+    # log psi in each log ratio, i.e. ratio errors of ~1e-4
+    tol = 1e-5 + 32 * 2.0 ** -24 * float(np.max(np.abs(la0)))
+    assert max_rel(e.cpu().numpy(), ref) < tol, tol
+
+
 @pytest.mark.parametrize("precision,tie,lptol", [("fp32", 1e-4, 1e-5), ("bf16x3", 1e-3, 1e-4)])
 def test_resconv_rows_metropolis_16x16(precision, tie, lptol, rng):
     lat, model, spec, theta = _resconv_rect(16, 16, 2, 16, precision, nup=128)
