@@ -480,6 +480,31 @@ def test_resconv_eloc_ising_single_flips(L, precision, tol, rng):
     assert (nconn.cpu().numpy() == L * L).all()
 
 
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-5), ("bf16x3", 1e-4)])
+@pytest.mark.parametrize("L0,ham", [(12, "j1j2"), (5, "ising")])
+def test_resconv_rows_eloc(L0, ham, precision, tol, rng):
+    """E_loc on the 16-wide row kernel, two configurations per workgroup: each
+    of the two proposal copies gets its own candidate, and a sample with an
+    odd number of valid candidates leaves the second copy idle in its last
+    batch."""
+    lat, model, spec, theta = _resconv_rect(L0, 16, 2, 16, precision,
+                                            nup=(L0 * 8) if ham == "j1j2" else None)
+    if ham == "j1j2":
+        H = qtx.operator.Heisenberg(J=[1, 0.5], n_neighbor=[1, 2], msr=True)
+        Ho = oop.heisenberg([L0, 16], [1, 0.5], [1, 2], True)
+        s = rand_spins(rng, 4, L0 * 16, nup=L0 * 8)
+    else:
+        H = qtx.operator.Ising(h=0.7, J=1.3)
+        Ho = oop.ising([L0, 16], h=0.7, J=1.3)
+        s = rand_spins(rng, 4, L0 * 16)
+    la = model.logpsi(to_dev(s))
+    e, nconn = model.eloc(H, to_dev(s), la)
+    ref = oop.oloc(s, lambda x: om.resconv_forward(spec, theta, x), Ho)
+    assert max_rel(e.cpu().numpy(), ref) < tol
+    seg = np.concatenate([oop.get_conn(sc, Hc)[0] for sc, Hc in oop.apply_off_diag(s, Ho).values()])
+    assert np.array_equal(nconn.cpu().numpy(), np.bincount(seg, minlength=4))
+
+
 def test_resconv_eloc_large_log_ratios(rng):
     """Connected configurations whose |log psi(s') - log psi(s)| passes 80
     take the FP64 exp in the E_loc kernel (FP32 overflows near 88).  The last
