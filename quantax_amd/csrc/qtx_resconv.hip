@@ -239,6 +239,7 @@ __device__ __forceinline__ int swz_chunk(int p, int j) {
 
 struct RcGeom {
   int L0, L1, N, NT;
+  double log_cn;  // log(C N): the head's normalisation
 };
 
 __device__ __forceinline__ int tap_pos(const RcGeom& g, int y, int x, int tap) {
@@ -478,15 +479,17 @@ __device__ __attribute__((always_inline)) inline void rc_forward(const RcGeom& g
       }
       __syncthreads();
       if (threadIdx.x < B) {
+        // waves' (max, sum) merged in FP32 (the sums are FP32 already): FP64
+        // exp / log here would hold the workgroup at the barrier below
         int b = threadIdx.x;
         float m = -INFINITY;
         for (int w = 0; w < W; ++w) m = fmaxf(m, red[(b * W + w) * 2]);
-        double ssum = 0.0;
+        float ssum = 0.0f;
         for (int w = 0; w < W; ++w) {
-          float mw = red[(b * W + w) * 2];
-          if (mw > -INFINITY) ssum += (double)red[(b * W + w) * 2 + 1] * exp((double)mw - (double)m);
+          const float mw = red[(b * W + w) * 2];
+          if (mw > -INFINITY) ssum += red[(b * W + w) * 2 + 1] * __expf(mw - m);
         }
-        lp_out[b] = (double)m + log(ssum) - log((double)C * (double)g.N);
+        lp_out[b] = (double)m + (double)logf(ssum) - g.log_cn;
       }
       __syncthreads();
     }
@@ -696,15 +699,17 @@ __device__ __attribute__((always_inline)) inline void rc_forward_rows(const RcGe
       }
       __syncthreads();
       if (threadIdx.x < B) {
+        // waves' (max, sum) merged in FP32 (the sums are FP32 already): FP64
+        // exp / log here would hold the workgroup at the barrier below
         int b = threadIdx.x;
         float m = -INFINITY;
         for (int w = 0; w < W; ++w) m = fmaxf(m, red[(b * W + w) * 2]);
-        double ssum = 0.0;
+        float ssum = 0.0f;
         for (int w = 0; w < W; ++w) {
-          float mw = red[(b * W + w) * 2];
-          if (mw > -INFINITY) ssum += (double)red[(b * W + w) * 2 + 1] * exp((double)mw - (double)m);
+          const float mw = red[(b * W + w) * 2];
+          if (mw > -INFINITY) ssum += red[(b * W + w) * 2 + 1] * __expf(mw - m);
         }
-        lp_out[b] = (double)m + log(ssum) - log((double)C * (double)g.N);
+        lp_out[b] = (double)m + (double)logf(ssum) - g.log_cn;
       }
       __syncthreads();
     }
@@ -1529,7 +1534,7 @@ static int rc_check(const qtx_resconv_desc* d) {
 
 static RcArgs rc_args(const RcLayout& L, const void* packed) {
   RcArgs a;
-  a.g = RcGeom{L.L0, L.L1, L.N, (L.N + 15) / 16};
+  a.g = RcGeom{L.L0, L.L1, L.N, (L.N + 15) / 16, std::log((double)L.C * (double)L.N)};
   a.nb = L.nb;
   a.gelu_exact = L.gelu_exact;
   a.packed = (const char*)packed;
@@ -1716,7 +1721,7 @@ int qtx_resconv_jacobian(void* stream, const qtx_resconv_desc* d, const void* pa
   if (ws == nullptr || ws_bytes < need)
     return fail(QTX_EWORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
   RcLayout L = rc_layout(d);
-  RcGeom g{L.L0, L.L1, L.N, (L.N + 15) / 16};
+  RcGeom g{L.L0, L.L1, L.N, (L.N + 15) / 16, std::log((double)L.C * (double)L.N)};
   hipStream_t st = as_stream(stream);
   const float* theta = (const float*)packed;  // raw theta copy at offset 0
   int64_t chunk = jac_chunk(ns);
