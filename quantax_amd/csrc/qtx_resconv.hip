@@ -140,7 +140,15 @@ __global__ void rc_pack_frag_kernel(const float* __restrict__ theta, int nb, cha
     reinterpret_cast<float*>(dst)[((int64_t)ct * KKS + ks) * WAVE + lane] = v;
   } else {
     int tap, c0;
+    // window layout: per K-step a plane of 64 x 16 B slot-0 fragments, then
+    // one of slot-1 fragments (16-byte lane stride for the LDS reads);
+    // other layouts: the two slots of a lane side by side
     __bf16* o = reinterpret_cast<__bf16*>(dst) + (((int64_t)ct * KKS + ks) * WAVE + lane) * 16;
+    __bf16* o1 = o + 8;
+    if (WIN) {
+      o = reinterpret_cast<__bf16*>(dst) + ((((int64_t)ct * KKS + ks) * 2) * WAVE + lane) * 8;
+      o1 = o + WAVE * 8;
+    }
     if (WIN && (ks % 2) == 1) {
       // tap (dy, 2) holds 16 of a K-step's 32 rows, so its three split
       // products share instructions: slot 0 of K-step 2 dy + 1 is
@@ -156,7 +164,7 @@ __global__ void rc_pack_frag_kernel(const float* __restrict__ theta, int nb, cha
         float v1 = 0.0f;
         if (dy == 0) v1 = W(c0 + q, up ? 5 : 2);
         else if (dy == 1 && !up) v1 = W(c0 + q, 8);
-        o[8 + q] = (__bf16)v1;
+        o1[q] = (__bf16)v1;
       }
       return;
     }
@@ -176,7 +184,7 @@ __global__ void rc_pack_frag_kernel(const float* __restrict__ theta, int nb, cha
       __bf16 hi = (__bf16)v;
       __bf16 lo = (__bf16)(v - (float)hi);
       o[q] = hi;
-      o[8 + q] = lo;
+      o1[q] = lo;
     }
   }
 }
@@ -761,21 +769,25 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
   const int b = wv / 4, y0 = RPW * (wv % 4);
   const float* pf = reinterpret_cast<const float*>(packed);
 
+  // the 16-byte chunks of column c (64 bytes per column) are XORed with
+  // (c >> 1) & 3: the eight lanes a ds_read_b128 serves per cycle read eight
+  // consecutive columns, which then fall on distinct 16-byte bank groups
+  // (with (c >> 2) & 3 columns c and c + 2 collide: the sweep is 6.6 % slower)
   int xoff[3];
 #pragma unroll
   for (int dx = 0; dx < 3; ++dx) {
     int xs = (x + dx + 15) & 15;
-    xoff[dx] = xs * 64 + ((gq ^ ((xs >> 2) & 3)) << 4);
+    xoff[dx] = xs * 64 + ((gq ^ ((xs >> 1) & 3)) << 4);
   }
   int khi[2], klo[2];
 #pragma unroll
   for (int pr = 0; pr < 2; ++pr) {
     int dx = (pr == 0) ? (gq >> 1) : 2;
-    int xs = (x + dx + 15) & 15, key = (xs >> 2) & 3, jh = gq & 1;
+    int xs = (x + dx + 15) & 15, key = (xs >> 1) & 3, jh = gq & 1;
     khi[pr] = xs * 64 + ((jh ^ key) << 4);
     klo[pr] = xs * 64 + (((2 + jh) ^ key) << 4);
   }
-  const int key0 = (x >> 2) & 3;
+  const int key0 = (x >> 1) & 3;
   const int st_f32 = x * 64 + ((gq ^ key0) << 4);
   const int st_hi = x * 64 + (((gq >> 1) ^ key0) << 4) + (gq & 1) * 8;
   const int st_lo = x * 64 + (((2 + (gq >> 1)) ^ key0) << 4) + (gq & 1) * 8;
@@ -866,9 +878,9 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
     } else {
 #pragma unroll
       for (int i = 0; i < KSW; ++i) {
-        const v8bf* src = reinterpret_cast<const v8bf*>(fr) + (i * WAVE + lane) * 2;
+        const v8bf* src = reinterpret_cast<const v8bf*>(fr) + 2 * i * WAVE + lane;
         wh[i] = src[0];
-        wl[i] = src[1];
+        wl[i] = src[WAVE];
       }
     }
     const float* bias = pf + bias_off + (int64_t)l * C;
