@@ -241,8 +241,11 @@ __device__ __forceinline__ void split_bf16x4(v4f v, v4bf& hi, v4bf& lo) {
 // ---------------------------------------------------------------- forward core
 template <int C>
 __device__ __forceinline__ int swz_chunk(int p, int j) {
+  // a ds_read_b128 is served eight lanes (128 bytes, 32 banks) at a time and
+  // the eight lanes read eight consecutive sites: the key makes their chunks
+  // fall on distinct 16-byte bank groups
   constexpr int CH = C / 4;
-  return j ^ ((p / (16 / CH)) & (CH - 1));
+  return j ^ ((p / (8 / CH)) & (CH - 1));
 }
 
 struct RcGeom {
@@ -527,12 +530,12 @@ __device__ __attribute__((always_inline)) inline void rc_forward_rows(const RcGe
   const int buf_bytes = cfg_bytes * B;
   const float* pf = reinterpret_cast<const float*>(packed);
 
-  // per-lane column constants (swizzle key of column c is (c >> 2) & 3)
+  // per-lane column constants (swizzle key of column c is (c >> 1) & 3)
   int xoff[3];
 #pragma unroll
   for (int dx = 0; dx < 3; ++dx) {
     int xs = (x + dx + 15) & 15;
-    xoff[dx] = xs * 64 + ((gq ^ ((xs >> 2) & 3)) << 4);
+    xoff[dx] = xs * 64 + ((gq ^ ((xs >> 1) & 3)) << 4);
   }
   int kdy[5], khi[5], klo[5];
 #pragma unroll
@@ -540,12 +543,12 @@ __device__ __attribute__((always_inline)) inline void rc_forward_rows(const RcGe
     int tap = 2 * ks + (gq >> 1);
     bool valid = tap < 9;
     int dy = valid ? tap / 3 : 1, dx = valid ? tap % 3 : 1;
-    int xs = (x + dx + 15) & 15, key = (xs >> 2) & 3, jh = gq & 1;
+    int xs = (x + dx + 15) & 15, key = (xs >> 1) & 3, jh = gq & 1;
     kdy[ks] = dy;
     khi[ks] = xs * 64 + ((jh ^ key) << 4);
     klo[ks] = xs * 64 + (((2 + jh) ^ key) << 4);
   }
-  const int key0 = (x >> 2) & 3;
+  const int key0 = (x >> 1) & 3;
   const int st_f32 = x * 64 + ((gq ^ key0) << 4);
   const int st_hi = x * 64 + (((gq >> 1) ^ key0) << 4) + (gq & 1) * 8;
   const int st_lo = x * 64 + (((2 + (gq >> 1)) ^ key0) << 4) + (gq & 1) * 8;
