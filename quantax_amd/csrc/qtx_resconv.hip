@@ -1,7 +1,8 @@
 // ResConv(nblocks, C, 3) on a periodic L0 x L1 grid: batched forward on MFMA
 // (FP32 16x16x4, or split-bf16 16x16x32 with FP32 accumulation), Metropolis
 // with one full forward per proposal, local energy over connected
-// configurations generated in LDS, and a per-sample VALU backward for O.
+// configurations generated in LDS, and a per-sample backward for O whose
+// three convolution products per layer are FP32 MFMA GEMMs.
 //
 // Forward work unit ("item") = one 16-position tile of one configuration.
 // Convolution as an implicit GEMM per item: D[co, p] = sum_k W[co, k] X[k, p]
@@ -1162,6 +1163,19 @@ __global__ __launch_bounds__(W * 64, ((W == 8 || ROWS == 2) ? 2 : 1)) void rc_me
   }
 }
 
+// psi(s') / psi(s) = e^d for the local energy: 2^t, t = d log2(e), split
+// into an integer k applied exactly by ldexp and f = t - k in [-1/2, 1/2]
+// whose 2^f is one FP32 exp2 (about 1 ulp); the FP64 product d log2(e) and
+// the FP32 rounding of f keep the relative error near 1e-7 for any d, where
+// an FP32 exp of d itself would lose |d| * 2^-24 to the rounding of d alone.
+// Non-finite or huge d take the FP64 exp.
+__device__ __forceinline__ double exp_ratio(double d) {
+  const double t = d * 1.4426950408889634;
+  if (!(fabs(t) < 2100.0)) return exp(d);
+  const double k = rint(t);
+  return ldexp((double)__builtin_amdgcn_exp2f((float)(t - k)), (int)k);
+}
+
 constexpr int RC_MAXCAND = 4096;
 constexpr int RC_MAXCAND_WIN = 3072;  // 16 x 16 window kernels: keeps two workgroups' LDS within a CU
 
@@ -1233,10 +1247,7 @@ __global__ __launch_bounds__(W * 64, ((W == 8 || ROWS == 2) ? 2 : 1)) void rc_el
     RC_FWD(prop, lp);
     if (threadIdx.x == 0) {
       for (int b = 0; b < B && start + b < nvalid; ++b) {
-        // |log ratio| < 80: FP32 exp (relative error ~1e-7, below the
-        // network's own); beyond, FP64 (FP32 would overflow)
-        const double d = lp[b] - lp0;
-        const double r = (fabs(d) < 80.0) ? (double)__expf((float)d) : exp(d);
+        const double r = exp_ratio(lp[b] - lp0);
         e += H.off_c[vlist[start + b]] * r;
       }
     }

@@ -115,16 +115,26 @@ def test_jacobian_sym_rbm(key, rng):
 
 @pytest.mark.parametrize("key", ["c4v_a1_z2", "c4v_b2_zm"])
 def test_oloc_sym(key, rng):
+    """E_loc over projected amplitudes.  Samples whose psi_sym vanishes (an
+    element of character -1 maps them to themselves; E_loc = psi'/0 is not
+    finite in the reference either) are not drawn: the first 20 samples with
+    |psi_sym| above 1e-10 of their largest image amplitude are compared, and
+    the vanishing ones are checked as vanishing on the device by
+    test_logpsi_sym_rbm."""
     lat, model, state, symm, W, b, fn = _rbm(key, nup=8)
     H = qtx.operator.Heisenberg(J=[1, 0.5], n_neighbor=[1, 2], msr=True)
     Ho = oop.heisenberg([4, 4], [1, 0.5], [1, 2], True)
-    s = _spins(rng, 20, nup=8)
+    cand = _spins(rng, 200, nup=8)
+    _, la = fn(cand)
+    big = np.array([om.rbm_logpsi(W, b, osym.images(x, symm.perm, symm.Z2_inversion))[1].max() for x in cand])
+    keep = np.isfinite(la) & (la - big > np.log(1e-10))
+    s = cand[keep][:20]
+    assert s.shape[0] == 20
     sd = torch.from_numpy(s).cuda()
-    psi = state(sd)
     e = H.Oloc(state, sd).cpu().numpy()
     ref = oop.oloc(s, fn, Ho)
-    ok = np.isfinite(ref)
-    assert max_rel(e[ok], ref[ok]) < 1e-10
+    assert np.all(np.isfinite(ref)) and np.all(np.isfinite(e))
+    assert max_rel(e, ref) < 1e-10
 
 
 @pytest.mark.parametrize("kind", ["flip", "exchange"])
@@ -164,7 +174,9 @@ def test_heisenberg4x4_symmetric_rbm_sr():
     """4x4 Heisenberg (Marshall sign, Sz = 0), RBM(16) float64 projected on
     the full space group and spin inversion (256 images): SR reaches the
     exact ground-state energy -44.913932833715506 (Pauli convention; the
-    oracle's ED value, pinned to tutorials/exact_diag.ipynb:191-192) to 1 %."""
+    oracle's ED value, pinned to tutorials/exact_diag.ipynb:191-192): the
+    mean of the last 20 steps is within 2.9e-5 of it on an MI355X (its Monte
+    Carlo error is of that order, so the bound is 2e-4)."""
     qtx.set_random_seed(0)
     qtx.sites.Square(4, Nparticles=(8, 8))
     H = qtx.operator.Heisenberg(msr=True)
@@ -178,7 +190,8 @@ def test_heisenberg4x4_symmetric_rbm_sr():
         drv.step()
     e = drv.energy.mean(20)
     e0 = -44.913932833715506
-    assert abs(e - e0) / abs(e0) < 1e-2, e
+    print(f"symmetric 4x4 energy {e:.10f}, relative error {abs(e - e0) / abs(e0):.3e}")
+    assert abs(e - e0) / abs(e0) < 2e-4, e
 
 
 def test_jacobian_sym_resconv(rng):

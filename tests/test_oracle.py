@@ -122,6 +122,47 @@ def test_eloc_weighted_mean_equals_rayleigh_quotient(which, rng):
     assert abs(lhs - rhs) < 1e-10 * max(1, abs(rhs))
 
 
+@pytest.mark.parametrize("reweight", [0.0, 0.5, 1.0, 2.0, 3.0])
+def test_reweight_factor_closed_form(reweight, rng):
+    """r = |psi|^(2 - n) / mean |psi|^(2 - n) (sampler/sampler.py:66-69) from
+    psi itself, without the oracle's max shift: identical, mean 1, r = 1 at
+    n = 2, and unchanged when every log psi moves by the same constant."""
+    la = rng.standard_normal(300) * 2.0
+    psi = np.exp(la)
+    direct = np.abs(psi) ** (2.0 - reweight)
+    direct /= direct.mean()
+    r = osm.reweight_factor(la, reweight)
+    assert np.max(np.abs(r - direct) / direct) < 1e-13
+    assert abs(r.mean() - 1.0) < 1e-14
+    assert np.array_equal(osm.reweight_factor(la + 700.0, reweight) > 0, np.ones(300, bool))
+    assert np.max(np.abs(osm.reweight_factor(la + 700.0, reweight) - r) / r) < 1e-12
+    if reweight == 2.0:
+        assert np.all(r == 1.0)
+
+
+def test_expectation_with_weights_equals_dense_moments(rng):
+    """Uniform samples of the whole basis (reweight 0) with the weights
+    |psi|^2 / mean |psi|^2: expectation() is <psi|H|psi>/<psi|psi> and its
+    variance <psi|H^2|psi>/<psi|psi> - <H>^2 of the dense H
+    (operator.py:564-595); the unweighted mean is not."""
+    N, op = 10, oop.heisenberg([10], [1.0, 0.3], [1, 2])
+    W = rng.standard_normal((8, N)) * 0.5
+    b = rng.standard_normal(8) * 0.3
+    basis, _ = ed.basis_states(N)
+    la = om.rbm_logpsi(W, b, basis)[1]
+    E = oop.oloc(basis, lambda x: om.rbm_logpsi(W, b, x), op)
+    m, v = oop.expectation(E, osm.reweight_factor(la, 0.0), return_var=True)
+    Hs, _ = ed.sparse_hamiltonian(op, N)
+    psi = np.exp(la - la.max())
+    hp = Hs @ psi
+    e_ref = psi @ hp / (psi @ psi)
+    v_ref = hp @ hp / (psi @ psi) - e_ref ** 2
+    assert abs(m - e_ref) < 1e-11 * abs(e_ref)
+    assert abs(v - v_ref) < 1e-9 * max(1.0, v_ref)
+    assert abs(oop.expectation(E) - e_ref) > 1e-3  # the weights matter here
+    assert abs(oop.expectation(E, osm.reweight_factor(la, 0.0)) - m) < 1e-14 * abs(m)
+
+
 def test_compiled_tables_equal_literal_application(rng):
     for op, N, nup in [(oop.heisenberg([4, 4], [1, 0.5], [1, 2], True), 16, 8), (oop.ising([10], 0.3, 2.0), 10, None),
                        (oop.heisenberg([6], [1.0, 0.2], [1, 2]), 6, None)]:
