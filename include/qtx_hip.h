@@ -222,6 +222,18 @@ size_t qtx_resconv_jacobian_ws_bytes(const qtx_resconv_desc* d, int64_t ns);
 int qtx_resconv_jacobian(void* stream, const qtx_resconv_desc* d, const void* packed, const int8_t* s,
                          int64_t ns, float* O, void* ws, size_t ws_bytes);
 
+/* The same O as qtx_resconv_jacobian with no workspace argument and no
+ * allocation: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The per-sample records live in a
+ * 128 MiB file-scope device array of the library, one per GPU, held until the
+ * process ends; samples run in launches of as many as it holds (e.g. 256 at
+ * 16 x 16, 8 blocks, 16 channels).  Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (different streams or threads) race.
+ * Returns QTX_EINVAL when one sample's records exceed the array. */
+int qtx_resconv_jacobian_noalloc(void* stream, const qtx_resconv_desc* d, const void* packed, const int8_t* s,
+                                 int64_t ns, float* O);
+
 /* ---------------------------------------------------------------- SR / MinSR
  * Column means of O (ns, np) in o_dtype, accumulated in float64
  * (the Omean of optimizer/sr.py:79-88). */

@@ -80,6 +80,7 @@ PROTOS = {
     "qtx_resconv_eloc": (C.c_int, [vp, C.POINTER(QtxResConvDesc), vp, C.POINTER(QtxHam), vp, vp, i64, vp, vp]),
     "qtx_resconv_jacobian_ws_bytes": (sz, [C.POINTER(QtxResConvDesc), i64]),
     "qtx_resconv_jacobian": (C.c_int, [vp, C.POINTER(QtxResConvDesc), vp, vp, i64, vp, vp, sz]),
+    "qtx_resconv_jacobian_noalloc": (C.c_int, [vp, C.POINTER(QtxResConvDesc), vp, vp, i64, vp]),
     "qtx_colmean": (C.c_int, [vp, C.c_int, vp, i64, i64, vp]),
     "qtx_ebar": (C.c_int, [vp, vp, vp, i64, vp, vp]),
     "qtx_obar_matvec": (C.c_int, [vp, C.c_int, vp, vp, vp, i64, i64, C.c_double, vp, C.c_double, vp]),

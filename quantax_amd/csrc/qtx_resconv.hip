@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -1763,6 +1764,62 @@ int qtx_resconv_jacobian(void* stream, const qtx_resconv_desc* d, const void* pa
       hipLaunchKernelGGL((rc_jacobian_kernel<32, 400>), dim3((unsigned)nc), dim3(JAC_THREADS), 0, st, g, L.nb,
                          L.gelu_exact, theta, s, s0, nc, L.nparams, O, (float*)ws);
     QTX_LAUNCH_CHECK();
+  }
+  return QTX_OK;
+}
+
+// ---------------------------------------------------------------- allocation-free Jacobian
+// The per-sample forward records of qtx_resconv_jacobian_noalloc live in this
+// file-scope device array (one copy per GPU, held until the process ends)
+// instead of a caller workspace; samples run in launches of as many as the
+// array holds.  Host code uses the per-device address from
+// hipGetSymbolAddress, never the array's name (a host-side stand-in).
+constexpr size_t RC_JAC_SCRATCH_BYTES = (size_t)128 << 20;
+static __device__ __align__(256) char scratch[RC_JAC_SCRATCH_BYTES];
+
+namespace {
+constexpr int RC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[RC_MAX_DEVICES] = {};
+
+// the calling thread's current device's copy of scratch
+int jac_scratch(void** out) {
+  int dev = 0;
+  QTX_HIP(hipGetDevice(&dev));
+  QTX_REQUIRE(dev >= 0 && dev < RC_MAX_DEVICES, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    QTX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    QTX_REQUIRE(p != nullptr, "hipGetSymbolAddress returned no address for the Jacobian scratch");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return QTX_OK;
+}
+}  // namespace
+
+int qtx_resconv_jacobian_noalloc(void* stream, const qtx_resconv_desc* d, const void* packed, const int8_t* s,
+                                 int64_t ns, float* O) {
+  int rc = rc_check(d);
+  if (rc) return rc;
+  QTX_REQUIRE(packed && ns >= 0 && (ns == 0 || (s && O)), "bad arguments");
+  void* ws = nullptr;
+  rc = jac_scratch(&ws);
+  if (rc) return rc;
+  if (ns == 0) return QTX_OK;
+  const size_t per_sample = qtx_resconv_jacobian_ws_bytes(d, 1);
+  if (per_sample > RC_JAC_SCRATCH_BYTES)
+    return fail(QTX_EINVAL, "one sample's Jacobian records need " + std::to_string(per_sample) +
+                                " bytes, more than the " + std::to_string(RC_JAC_SCRATCH_BYTES) +
+                                "-byte scratch of the allocation-free entry point");
+  const int64_t per_launch = (int64_t)(RC_JAC_SCRATCH_BYTES / per_sample);
+  for (int64_t s0 = 0; s0 < ns; s0 += per_launch) {
+    const int64_t nc = (ns - s0) < per_launch ? (ns - s0) : per_launch;
+    // the kernel indexes samples from the start of the pointers it is given
+    rc = qtx_resconv_jacobian(stream, d, packed, s + s0 * (int64_t)d->L0 * d->L1, nc,
+                              O + s0 * qtx_resconv_nparams(d), ws, per_launch * per_sample);
+    if (rc) return rc;
   }
   return QTX_OK;
 }
