@@ -38,23 +38,31 @@ def parse():
     p.add_argument("--sweep-steps", type=int, default=None,
                    help="proposals per chain per sweep (default 2N; 8 for c4)")
     p.add_argument("--thermal-sweeps", type=int, default=None, help="thermalisation sweeps (default 1; 0 for c4)")
-    p.add_argument("--solver", default="eigh", choices=["eigh", "eigh-direct", "chol"],
+    p.add_argument("--solver", default=None, choices=["eigh", "eigh-direct", "chol"],
                    help="eigh: soft pseudo-inverse (partial fractions when the cutoff allows); "
-                        "eigh-direct: the same through dsyevd; chol: diagonal-shift Cholesky")
+                        "eigh-direct: the same through dsyevd; chol: diagonal-shift Cholesky "
+                        "(default: the configuration's own, chol for c1/c2, eigh for c3/c4)")
     p.add_argument("--rtol", type=float, default=1e-4, help="soft pseudo-inverse cutoff of the eigh solver")
-    p.add_argument("--lr", type=float, default=1e-3)
+    p.add_argument("--lr", type=float, default=None, help="learning rate (default: the configuration's own)")
+    p.add_argument("--no-ref-solve", action="store_true",
+                   help="skip the solve with the reference's default cutoff rtol = 1e-12 (dsyevd route)")
+    p.add_argument("--profile-kernels", action="store_true",
+                   help="kernel profiling run: one sweep of --sweep-steps proposals (default 16) and E_loc of "
+                        "--profile-eloc-samples samples, nothing else; prints one JSON line of HIP-event times")
+    p.add_argument("--profile-eloc-samples", type=int, default=2048)
     p.add_argument("--precision-breakdown", action="store_true",
                    help="also attribute the mixed update-vector error to E_loc and to the Gram")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-serve", action="store_true", help="skip the served job stream")
     p.add_argument("--serve-jobs", type=int, default=64)
-    p.add_argument("--serve-steps", type=int, default=10, help="MinSR steps per served job (the stream's jobs have 50)")
+    p.add_argument("--serve-steps", type=int, default=50, help="MinSR steps per served job")
     p.add_argument("--serve-gap", type=float, default=0.25, help="mean Poisson inter-arrival time (s)")
     p.add_argument("--serve-max-active", type=int, default=12,
                    help="cap on concurrently served jobs (0: no cap); the cap lowers median latency")
-    p.add_argument("--serve-seq-steps", type=int, default=3,
-                   help="steps per job timed for the sequential baseline (the rest extrapolated)")
+    p.add_argument("--serve-seq-steps", type=int, default=0,
+                   help="steps per job timed for the sequential baselines (0: every step of every job; "
+                        "otherwise the rest is extrapolated at the job's later-step cost)")
     p.add_argument("--calib-copy-mb", type=int, default=0,
                    help="run a device-to-device copy of this many MiB first (HBM counter calibration)")
     return p.parse_args()
@@ -63,29 +71,31 @@ def parse():
 CONFIGS = {
     # name: (L, lattice kind, model kind, model args, ns, J, n_neighbor)
     "c3": dict(L=16, model="resconv", nblocks=8, channels=16, ns=16384, J=[1.0, 0.5], nn=[1, 2], msr=True,
+               solver="eigh", lr=1e-3,
                workload="J1-J2 16x16 (J2=0.5, MSR, Sz=0), ResConv(8 blocks, 16 ch), 16384 chains, MinSR eigh"),
-    "c2": dict(L=10, model="rbm", features=400, ns=4096, J=[1.0], nn=[1], msr=True,
-               workload="Heisenberg 10x10 (MSR, Sz=0), RBM_Dense(alpha=4), 4096 chains, MinSR"),
-    "c1": dict(L=16, model="rbm1d", features=16, ns=1024,
-               workload="TFIM chain L=16 (h=1), RBM_Dense(16) float64, 1024 chains, SR"),
+    "c2": dict(L=10, model="rbm", features=400, ns=4096, J=[1.0], nn=[1], msr=True, solver="chol", lr=1e-2,
+               workload="Heisenberg 10x10 (MSR, Sz=0), RBM_Dense(alpha=4) float32, 4096 chains, MinSR with "
+                        "diagonal shift 1e-4 (Cholesky)"),
+    "c1": dict(L=16, model="rbm1d", features=16, ns=1024, solver="chol", lr=1e-2,
+               workload="TFIM chain L=16 (h=1), RBM_Dense(16) float64, 1024 chains, SR with diagonal shift 1e-3"),
     # the large case: sampling is cut to 8 proposals per chain and no
     # thermalisation (a full 2N sweep is 6 PFLOP of forwards); E_loc, O, the
     # 32768 x 32768 Gram and the solve run in full
     "c4": dict(L=20, model="resconv", nblocks=16, channels=32, ns=32768, J=[1.0, 0.5], nn=[1, 2], msr=True,
-               sweep_steps=8, thermal_sweeps=0, nconn_est=800,
+               sweep_steps=8, thermal_sweeps=0, nconn_est=800, solver="eigh", lr=1e-3,
                workload="J1-J2 20x20 (J2=0.5, MSR, Sz=0), ResConv(16 blocks, 32 ch), 32768 chains, MinSR "
                         "(T 32768^2); sampling cut to 8 proposals per chain, no thermalisation"),
 }
 
 
 # HBM traffic of one rc_metropolis_kernel launch at the 16x16 workload (mixed,
-# 512 steps, 16384 chains), from two rocprofv3 --pmc runs of this script
-# (FETCH_SIZE, then WRITE_SIZE, kernel-include-regex 'rc_metropolis|copyBuffer',
-# --steps 1 --warmup 0 --mode mixed --no-cpu-baseline --no-serve
-# --thermal-sweeps 0 --calib-copy-mb 256), each scaled by what the counter
-# reads for the 256 MiB device copy: FETCH 131092.7 KiB (scale 262144 /
-# 131092.7), WRITE 262144.0 KiB (scale 1).
-PMC_C3_MIXED = {"fetch_kib": 134351.8, "write_kib": 4352.0, "fetch_scale": 262144.0 / 131092.7, "write_scale": 1.0}
+# 512 steps, 16384 chains): NOT measured by this run.  It comes from two
+# rocprofv3 --pmc runs of this script (FETCH_SIZE, then WRITE_SIZE), each
+# scaled by what the counter reads for a 256 MiB device copy; the commands,
+# raw counts and the calibration are in the profile named below, and the
+# bench line reports the figure with that source.
+PMC_C3_MIXED = {"fetch_kib": 134351.8, "write_kib": 4352.0, "fetch_scale": 262144.0 / 131092.7, "write_scale": 1.0,
+                "source": "profiles/rc_metropolis_16x16_c16_nb8_hbm_traffic.md"}
 
 
 def conv_flops_per_forward(nblocks, C, N):
@@ -211,6 +221,8 @@ def run_mode(cfg, mode, args, torch, qtx):
             out["per_step_gram_err"] = errs
             out["gram"] = gram_rate(last["O"], last["mean"], torch)
             out["precision"] = precision_report(cfg, state, H, solver, last, torch, qtx, args.precision_breakdown)
+            if not args.no_ref_solve:
+                out["reference_default_solve"] = reference_default_solve(last, solver, torch, qtx)
         else:
             out["gram"] = gram_report(last, solver, torch, qtx, opt._ws)
     return out
@@ -253,6 +265,29 @@ def precision_report(cfg, state, H, solver, last, torch, qtx, breakdown=False):
     if breakdown:
         rep["err_from_eloc_only"] = rel(qtx.optimizer.solve_step(O, mean, eb_m, ref_solver))
         rep["err_from_gram_only"] = rel(qtx.optimizer.solve_step(O, mean, eb_p, solver))
+    return rep
+
+
+def reference_default_solve(last, solver, torch, qtx):
+    """The last step's solve with the reference's default cutoff rtol = 1e-12
+    (``optimizer/solver.py:12-21``): below 1e-5 lambda_max, so it takes the
+    dsyevd route; HIP events around the call, the step's own O and Ebar."""
+    ref = qtx.optimizer.Solver(solver.mode, solver.rshift, solver.ashift, 1e-12, 0.0, solver.gram)
+    ms = []
+    for _ in range(2):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        qtx.optimizer.solve_step(last["O"], last["mean"], last["ebar"], ref)
+        b.record()
+        torch.cuda.synchronize()
+        ms.append(a.elapsed_time(b))
+    info = qtx.optimizer.last_solve_info()
+    torch.cuda.empty_cache()
+    rep = {"rtol": 1e-12, "gram": solver.gram, "ms": round(min(ms), 1), "path": info["path"],
+           "fallback": info["fallback"], "note": "not part of the timed step, which uses rtol = "
+           f"{solver.rtol:g} (partial-fraction route)"}
+    print(f"[bench] reference-default solve (rtol 1e-12): {rep['ms']} ms via {rep['path']}", file=sys.stderr,
+          flush=True)
     return rep
 
 
@@ -307,36 +342,56 @@ def gram_report(last, solver, torch, qtx, ws_solve):
 
 def run_serving(args, torch):
     """The served job stream (64 J1-J2 RBM jobs, Poisson arrivals) against the
-    same jobs run one after another through the single-job path with the FP64
-    Gram; per-job cost is stationary, so the sequential total is extrapolated
-    from a few timed steps per job."""
+    same jobs run one after another through the single-job path, once with
+    the FP64 Gram (the plain path) and once with the same split-bf16 Gram as
+    the server, so batching and precision gains are reported apart.  Every
+    step of every job is timed unless --serve-seq-steps caps it."""
     from quantax_amd import serve
 
     specs = serve.c5_stream(args.serve_jobs, mean_interarrival=args.serve_gap, nsteps=args.serve_steps, seed=5)
     log = lambda m: print(f"[bench] serve: {m}", file=sys.stderr, flush=True)
     srv = serve.Server(progress=log, max_active=args.serve_max_active or None)
     rep = srv.run(specs)
+    rounds = srv.round_summary()
     srv.close()
-    log(f"served: {rep.summary()}")
-    seq = serve.run_sequential(specs, gram="fp64", nsteps=args.serve_seq_steps, progress=log)
-    log(f"sequential: {seq.summary()}")
-    s, q = rep.summary(), seq.summary()
+    log(f"served: {rep.summary()}, rounds: {rounds}")
+    nseq = args.serve_seq_steps or None
+    seq = {}
+    for gram in ("fp64", "bf16x3"):
+        q = serve.run_sequential(specs, gram=gram, nsteps=nseq, progress=log)
+        seq[gram] = q.summary()
+        log(f"sequential ({gram} Gram): {seq[gram]}")
+    how = ("every step of every job timed" if nseq is None else
+           f"{nseq} steps per job timed, the rest extrapolated at the job's later-step cost")
+    s = rep.summary()
     return {
         "workload": (f"{args.serve_jobs} J1-J2 RBM_Dense jobs (L in 6..16, alpha in 1,2,4, ns in 1024..8192, "
                      f"J2 in [0, 0.6]), Poisson arrivals, mean gap {args.serve_gap} s, {args.serve_steps} MinSR "
                      f"steps each (split-bf16 Gram, soft pseudo-inverse rtol 1e-4, lr 1e-2), at most "
                      f"{args.serve_max_active or 'unlimited'} jobs in flight"),
         "served": s,
-        "sequential_plain": {**q, "note": f"single-job path, FP64 Gram, {args.serve_seq_steps} steps per job timed, "
-                                          "rest extrapolated at the job's later-step cost"},
-        "served_over_sequential": round(s["steps_per_s"] / q["steps_per_s"], 3),
+        "rounds": {**rounds, "note": "per round: host time enqueueing the three grouped launches and the per-job "
+                                     "Ebar/Omean/update launches; host time inside qtx_minsr_solve_many "
+                                     "(its enqueues and the readbacks it waits on); device time of the grouped "
+                                     "launches (HIP events)"},
+        "sequential_plain": {**seq["fp64"], "note": f"single-job path, FP64 Gram, {how}; latency = the job's own run"},
+        "sequential_mixed": {**seq["bf16x3"], "note": f"single-job path, split-bf16 Gram (as served), {how}"},
+        "served_over_sequential": round(s["steps_per_s"] / seq["fp64"]["steps_per_s"], 3),
+        "served_over_sequential_mixed": round(s["steps_per_s"] / seq["bf16x3"]["steps_per_s"], 3),
     }
 
 
 def cpu_baseline(cfg, seconds, sweep_steps):
-    """Time the float64 oracle on a bounded sample of the workload on the host
-    cores and convert it to VMC steps/s (forwards dominate: sweep + E_loc)."""
+    """CPU baseline on the host cores, rank 0: the oracle's FP32 ResConv
+    forward through torch's CPU convolution (oracle/cpu_resconv.py, the kind
+    of compiled CPU convolution a JAX-CPU run of the reference uses), timed on
+    a bounded sample of the workload's configurations and converted to VMC
+    steps/s by the forwards one step needs (sweep proposals + connected
+    configurations; the forwards are >99 % of a step's arithmetic).  The
+    float64 numpy restatement is timed beside it as a secondary figure."""
     import oracle.models as om
+    from oracle import cpu_resconv
+    import torch
 
     cores = len(os.sched_getaffinity(0))
     rng = np.random.default_rng(0)
@@ -346,21 +401,75 @@ def cpu_baseline(cfg, seconds, sweep_steps):
     N = L * L
     spec = om.ResConvSpec(cfg["nblocks"], cfg["channels"], (L, L))
     theta = spec.he_normal(rng).astype(np.float64)
-    batch = 16
-    s = (2 * rng.integers(0, 2, size=(batch, N)) - 1).astype(np.int8)
-    n = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
-        om.resconv_forward(spec, theta, s)
-        n += batch
-    dt = time.perf_counter() - t0
-    fwd_per_s = n / dt
     ns = cfg["ns"]
     # sweep proposals + connected configurations per sample (~550 at 16x16, ~800 at 20x20)
     fwd_per_step = ns * sweep_steps + ns * cfg.get("nconn_est", 550)
-    return dict(value=fwd_per_s / fwd_per_step, unit="VMC steps/s (extrapolated)", cores=cores, kind="port",
-                sample=f"{n} ResConv forwards of the float64 numpy oracle in {dt:.1f} s on {cores} cores "
-                       f"({fwd_per_s:.1f} forwards/s), scaled by {fwd_per_step} forwards per VMC step")
+
+    def rate(fn, batch, budget):
+        s = (2 * rng.integers(0, 2, size=(batch, N)) - 1).astype(np.int8)
+        fn(s)  # warm-up (allocator, thread pool)
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < budget:
+            fn(s)
+            n += batch
+        return n, time.perf_counter() - t0
+
+    threads = torch.get_num_threads()
+    torch.set_num_threads(cores)
+    try:
+        n, dt = rate(lambda s: cpu_resconv.resconv_forward_f32(spec, theta, s), 256, seconds)
+    finally:
+        torch.set_num_threads(threads)
+    fwd = n / dt
+    n64, dt64 = rate(lambda s: om.resconv_forward(spec, theta, s), 16, max(3.0, seconds / 3))
+    fwd64 = n64 / dt64
+    flops = conv_flops_per_forward(cfg["nblocks"], cfg["channels"], N)
+    return dict(value=fwd / fwd_per_step, unit="VMC steps/s (extrapolated)", cores=cores, kind="port",
+                sample=f"{n} ResConv forwards in FP32 through torch's CPU conv2d (circular padding, "
+                       f"oracle/cpu_resconv.py) in {dt:.1f} s on {cores} threads: {fwd:.0f} forwards/s = "
+                       f"{fwd * flops / 1e9:.1f} GFLOP/s, scaled by {fwd_per_step} forwards per VMC step",
+                secondary={"kind": "float64 numpy oracle (oracle/models.py)", "forwards_per_s": round(fwd64, 1),
+                           "value": fwd64 / fwd_per_step, "gflops": round(fwd64 * flops / 1e9, 2)})
+
+
+def dtype_label(cfg, mode):
+    """Arithmetic types of the step: network, Gram, solve."""
+    if cfg["model"] == "rbm1d":
+        return "fp64"  # float64 RBM; the SR branch (ns >= np) forms S in FP64
+    net = "bf16x3" if (mode == "mixed" and cfg["model"] == "resconv") else "fp32"
+    gram = "bf16x3" if mode == "mixed" else "fp64"
+    return f"{net} network + {gram} Gram + fp64 solve"
+
+
+def profile_kernels(cfg, args, torch, qtx):
+    """For rocprofv3 runs: build the configuration's mixed (or --mode plain)
+    ResConv problem with no thermalisation, run one sweep of --sweep-steps
+    proposals (16 by default) over all chains and E_loc of the first
+    --profile-eloc-samples samples; print the HIP-event times."""
+    mode = "plain" if args.mode == "plain" else "mixed"
+    args.thermal_sweeps = 0
+    if args.sweep_steps is None:
+        args.sweep_steps = 16
+    lat, H, state, sampler, solver = build_problem(cfg, mode, args, qtx)
+    samples = sampler.sweep()
+    torch.cuda.synchronize()
+    ms_sweep = sampler.kernel_events[-1][0].elapsed_time(sampler.kernel_events[-1][1])
+    n = min(args.profile_eloc_samples, cfg["ns"])
+    s, la = samples.spins[:n].contiguous(), samples.psi.logabs[:n].contiguous()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    e, nc = state.model.eloc(H, s, la)
+    b.record()
+    torch.cuda.synchronize()
+    N = cfg["L"] ** 2
+    fwd = conv_flops_per_forward(cfg["nblocks"], cfg["channels"], N)
+    ms_e = a.elapsed_time(b)
+    nconn = float(nc.float().mean().item())
+    print(json.dumps({"profile": args.config, "mode": mode, "sweep_steps": args.sweep_steps, "chains": cfg["ns"],
+                      "metropolis_ms": round(ms_sweep, 3),
+                      "metropolis_tflops": round(cfg["ns"] * args.sweep_steps * fwd / ms_sweep / 1e9, 1),
+                      "eloc_samples": n, "eloc_ms": round(ms_e, 3), "mean_nconn": round(nconn, 1),
+                      "eloc_tflops": round(n * nconn * fwd / ms_e / 1e9, 1)}))
 
 
 def main():
@@ -383,6 +492,13 @@ def main():
         print(f"[bench] calibration copy of {args.calib_copy_mb} MiB done", file=sys.stderr, flush=True)
         del src, dst
     cfg = CONFIGS[args.config]
+    if args.solver is None:
+        args.solver = cfg["solver"]
+    if args.lr is None:
+        args.lr = cfg["lr"]
+    if args.profile_kernels:
+        profile_kernels(cfg, args, torch, qtx)
+        return
     if args.sweep_steps is None:
         args.sweep_steps = cfg.get("sweep_steps")
     if args.thermal_sweeps is None:
@@ -421,6 +537,8 @@ def main():
             traffic = int(1024 * (p["fetch_kib"] * p["fetch_scale"] + p["write_kib"] * p["write_scale"]))
         roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(ach / peak, 4), "traffic": traffic,
+                "traffic_source": None if traffic is None else
+                "not measured in this run: rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE of this kernel, " + PMC_C3_MIXED["source"],
                 "hbm_gbps": None if traffic is None else round(traffic / t / 1e9, 3),
                 "hbm_peak_gbps": HBM_PEAK_GBS,
                 "traffic_algorithmic": cfg["ns"] * sweep_steps * 16 + 2 * cfg["ns"] * N + 24 * cfg["ns"],
@@ -439,8 +557,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x3+fp32+fp64" if main_mode == "mixed" else "fp32+fp64",
-        "data": "synthetic (He-normal ResConv weights, Philox-seeded chains)",
+        "dtype": dtype_label(cfg, main_mode),
+        "data": ("synthetic (He-normal ResConv weights, Philox-seeded chains)" if cfg["model"] == "resconv" else
+                 "synthetic (RBM_Dense weights N(0, 0.01^2), Philox-seeded chains)"),
         "config": {"workload": cfg["workload"], "mode": main_mode, "nsamples": cfg["ns"],
                    "sweep_steps": sweep_steps, "solver": args.solver, "rtol": args.rtol, "lr": args.lr,
                    "nparams": r["nparams"]},
@@ -461,6 +580,8 @@ def main():
         out["precision"]["per_step_update_vector_rel_err_gram"] = r.get("per_step_gram_err")
     if "gram" in r:
         out["gram"] = r["gram"]
+    if "reference_default_solve" in r:
+        out["reference_default_solve"] = r["reference_default_solve"]
     if cfg["model"] == "resconv":
         # fused E_loc: one forward per valid connected configuration; reads the
         # spins and log psi, writes E_loc and the counts (compute-bound)

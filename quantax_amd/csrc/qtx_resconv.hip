@@ -742,16 +742,16 @@ __device__ __attribute__((always_inline)) inline void rc_forward_rows(const RcGe
 // residual) are compile-time inside a layer (one branch per layer), and the
 // rows are software-pipelined: row i's MFMA chain and row i - 1's epilogue
 // (residual, GELU, bf16 split, LDS store) are one basic block, so VALU and
-// MFMA work of the same wave can overlap.  Layer l + 1's weight fragments
-// travel HBM/L2 -> LDS by DMA while layer l runs (wst: two stages of
-// RC_WSTAGE bytes) and are read from LDS at the layer start.  The residual
-// stream x (FP32, 16 bytes per lane and row) lives in wst after the stages,
-// in slots private to each lane, which keeps the pipelined loop in registers.
-// With B = 1 a workgroup needs 72 KB of LDS, so two run on a CU: each SIMD
-// then holds one wave of each, and their per-layer barriers are independent,
-// so one wave's MFMA phase can overlap the other's VALU phase.
+// MFMA work of the same wave can overlap.  Every wave reads layer l's weight
+// fragments from one LDS stage (wst, RC_WSTAGE bytes) into registers at the
+// layer start; after a barrier, layer l + 1's fragments travel HBM/L2 -> LDS
+// by DMA into the same stage while layer l runs.  The residual stream x
+// (FP32, 4 values per lane and row) stays in registers.  With B = 1 a
+// workgroup needs 44 KB of LDS and at most 168 VGPRs per wave, so three run
+// on a CU: each SIMD holds one wave of each, and their per-layer barriers are
+// independent, so one wave's MFMA phase can overlap the others' VALU phases.
 constexpr int RC_WSTAGE = 12288;  // one layer's fragments: bf16x3 6 x 64 x 32 B; FP32 9216 B
-constexpr int rc_win_lds(int B) { return 2 * RC_WSTAGE + B * 16 * 64 * 16; }  // weight stages + residual slots
+constexpr int rc_win_lds(int B) { return RC_WSTAGE; }  // one weight stage
 typedef __attribute__((address_space(3))) void rc_lds_void;
 
 template <bool GE>
@@ -828,17 +828,17 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
     }
     return F;
   };
-  // layer l's fragment block -> weight stage l & 1, frag_bytes / 1024 wave-wide DMA instructions
+  // layer l's fragment block -> the weight stage, frag_bytes / 1024 wave-wide DMA instructions
   auto stage_weights = [&](int l) {
     const char* src = packed + off_frag + (int64_t)l * frag_bytes;
-    char* dst = wst + (l & 1) * RC_WSTAGE;
+    char* dst = wst;
     const int nchunk = (int)(frag_bytes >> 10);
     for (int c = wv; c < nchunk; c += W)
       __builtin_amdgcn_global_load_lds((const void*)(src + c * 1024 + lane * 16), (rc_lds_void*)(dst + c * 1024), 16,
                                        0, 0);
   };
 
-  v4f* resv = reinterpret_cast<v4f*>(wst + 2 * RC_WSTAGE) + (b * 16 + y0) * WAVE + lane;  // row i: resv[i * WAVE]
+  v4f resr[RPW];  // residual stream of the wave's rows, MFMA output layout
   stage_weights(1);
   // ---- layer 0 on VALU
   {
@@ -863,7 +863,7 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
         for (int tap = 0; tap < 9; ++tap) a += W0[co * 9 + tap] * sv[tap];
         z[q] = gelu_f(a, gelu_exact);
       }
-      resv[i * WAVE] = v4f{spin, spin, spin, spin};
+      resr[i] = v4f{spin, spin, spin, spin};
       store4(act + b * CFGB + y * ROWB, z);
     }
   }
@@ -876,7 +876,7 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
     float wf[(MODE == QTX_CONV_FP32) ? KSW : 1];
     v8bf wh[(MODE == QTX_CONV_FP32) ? 1 : KSW];
     v8bf wl[(MODE == QTX_CONV_FP32) ? 1 : KSW];
-    const char* fr = wst + (l & 1) * RC_WSTAGE;
+    const char* fr = wst;
     if constexpr (MODE == QTX_CONV_FP32) {
 #pragma unroll
       for (int i = 0; i < KSW; ++i) wf[i] = reinterpret_cast<const float*>(fr)[i * WAVE + lane];
@@ -888,6 +888,10 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
         wl[i] = src[WAVE];
       }
     }
+    // every wave holds this layer's fragments: the next layer's travel into
+    // the stage while this one runs
+    __syncthreads();
+    if (l + 1 < 2 * nb) stage_weights(l + 1);
     const float* bias = pf + bias_off + (int64_t)l * C;
     const v4f bias4 = v4f{bias[4 * gq], bias[4 * gq + 1], bias[4 * gq + 2], bias[4 * gq + 3]};
     Frag Fm = load_row(in, y0 - 1);
@@ -941,7 +945,6 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
 
   int cur = 0;
   for (int l = 1; l < 2 * nb - 1; ++l) {
-    stage_weights(l + 1);  // stage (l + 1) & 1 was last read at the start of layer l - 1
     const char* in = act + cur * BUFB + b * CFGB;
     char* out = act + (cur ^ 1) * BUFB + b * CFGB;
     const bool is_conv2 = (l % 2) == 1;
@@ -952,8 +955,8 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
       run_layer(l, in, [&](int i, v4f acc) {
         v4f o;
         if constexpr (C2) {
-          v4f xr = acc + resv[i * WAVE];
-          resv[i * WAVE] = xr;
+          v4f xr = acc + resr[i];
+          resr[i] = xr;
 #pragma unroll
           for (int q = 0; q < 4; ++q) o[q] = gelu_t<GE>(xr[q] * scale);
         } else {
@@ -984,7 +987,7 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
     float m_l = -INFINITY, s_l = 0.0f;
     run_layer(l, in, [&](int i, v4f acc) {
       float hv[4];
-      const v4f r = resv[i * WAVE];
+      const v4f r = resr[i];
 #pragma unroll
       for (int q = 0; q < 4; ++q) hv[q] = (acc[q] + r[q]) * scale;
       const float mn = fmaxf(m_l, fmaxf(fmaxf(hv[0], hv[1]), fmaxf(hv[2], hv[3])));
@@ -1016,6 +1019,247 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
   }
 }
 
+// Split-bf16 forward for C = 32 on grids of up to 400 sites (the 20 x 20
+// benchmark lattice): one configuration per workgroup of W = 4 waves, two
+// workgroups per CU (2 waves per SIMD, at most 256 VGPRs each).  Items are
+// 16-site tiles in row-major order; wave w owns tiles w, w + 4, ... (at most
+// MAXT = 7 of the 25 of 20 x 20) and computes both 16-channel output tiles
+// (CTW = 2) of each, so one B fragment read from LDS feeds six MFMAs.  Per
+// layer the K loop runs over the 9 taps outside the tile loop: the A
+// fragments of one tap (2 output tiles x hi/lo, 16 VGPRs, prefetched one tap
+// ahead from L2) serve every tile of the wave, and the wave's 2 x MAXT
+// accumulators are independent MFMA chains.  Activations sit once in LDS
+// (bf16 hi/lo, 128 bytes per site, 16-byte chunks XORed with site & 7); a
+// layer computes into registers and its outputs replace its inputs between
+// two barriers, which halves the LDS of a ping-pong pair so two workgroups
+// fit on a CU.  The B fragment of (tap, tile) is found through a table of
+// byte offsets of the tap's source site (swizzle key in bits 4-6), built once
+// per launch.  The residual stream stays in registers in the MFMA output
+// layout.  CTW = 1 (six waves, wave pairs sharing tiles, three waves per
+// SIMD) ran the 20 x 20 sweep at 186-221 TFLOP/s against 301 for CTW = 2:
+// it reads every B fragment twice and issues three MFMAs per LDS round trip.
+constexpr int RC32_MAXN = 400;
+constexpr int RC32_CTW = 2;  // output tiles per wave
+constexpr int rc32_tab_bytes(int nmax) { return 9 * nmax * 2; }
+
+template <int MAXT, int W, int CTW, bool GE>
+__device__ __attribute__((always_inline)) inline void rc_forward_t32(
+    const RcGeom& g, int nb, const char* __restrict__ packed, int64_t off_l0w, int64_t off_l0b,
+    int64_t bias_off, int64_t off_frag, int64_t frag_bytes, const int8_t* sp, char* act, const uint16_t* tab,
+    float* red, double* lp_out) {
+  // CTW output tiles per wave: 1 (wave pairs share tiles, ct = w & 1) or 2
+  constexpr int C = 32, NP = (CTW == 1) ? W / 2 : W;  // NP groups of waves own disjoint tiles
+  const int lane = threadIdx.x % WAVE;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+  const int gq = lane / 16, x16 = lane % 16;
+  const int ct0 = (CTW == 1) ? (wv & 1) : 0, m = (CTW == 1) ? (wv >> 1) : wv;
+  const int N = g.N, NT = g.NT;
+  const int nti = (NT - m + NP - 1) / NP;  // tiles of this wave (wave-uniform)
+  const float* pf = reinterpret_cast<const float*>(packed);
+  // the B fragment of lane group gq is chunk gq (hi) / 4 + gq (lo)
+  const uint32_t xhi = (uint32_t)gq << 4, xlo = (uint32_t)(gq ^ 4) << 4;
+  // output stores: channel 16 ct + 4 gq + i of site p lies in chunk
+  // 2 ct + gq / 2 (hi) or 4 + 2 ct + gq / 2 (lo), half gq & 1; the key p & 7
+  // equals lane & 7 for every tile (tiles start at multiples of 16)
+  const int key = lane & 7;
+  int st_hi[CTW], st_lo[CTW];
+#pragma unroll
+  for (int j = 0; j < CTW; ++j) {
+    const int ct = ct0 + j;
+    st_hi[j] = x16 * 128 + (((2 * ct + (gq >> 1)) ^ key) << 4) + (gq & 1) * 8;
+    st_lo[j] = x16 * 128 + (((4 + 2 * ct + (gq >> 1)) ^ key) << 4) + (gq & 1) * 8;
+  }
+  auto site = [&](int k) { return 16 * (m + NP * k) + x16; };  // may be >= N
+  auto store4 = [&](int k, int j, v4f v) {
+    const int t = m + NP * k;
+    if (16 * t + x16 < N) {
+      v4bf hi, lo;
+      split_bf16x4(v, hi, lo);
+      char* base = act + t * 16 * 128;
+      *reinterpret_cast<v4bf*>(base + st_hi[j]) = hi;
+      *reinterpret_cast<v4bf*>(base + st_lo[j]) = lo;
+    }
+  };
+  // A fragments (hi, lo) of (layer, tap) for this wave's output tiles
+  auto load_a = [&](int l, int ks, v8bf (&a)[CTW][2]) {
+#pragma unroll
+    for (int j = 0; j < CTW; ++j) {
+      const v8bf* q = reinterpret_cast<const v8bf*>(packed + off_frag + (int64_t)l * frag_bytes) +
+                      (((ct0 + j) * 9 + ks) * WAVE + lane) * 2;
+      a[j][0] = q[0];
+      a[j][1] = q[1];
+    }
+  };
+
+  v4f res[MAXT][CTW];
+
+  // ---- layer 0: conv (Cin = 1) of s / sqrt(2) on VALU, GELU, into act
+  {
+    const float* W0 = pf + off_l0w;
+    const float* b0 = pf + off_l0b;
+#pragma unroll
+    for (int k = 0; k < MAXT; ++k) {
+      if (k < nti) {
+        const int p = site(k);
+        const int pc = p < N ? p : N - 1;
+        float sv[9];
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) sv[tap] = (float)sp[tab[tap * RC32_MAXN + pc] >> 7] * 0.70710678118654752f;
+        const float spin = (float)sp[pc];
+#pragma unroll
+        for (int j = 0; j < CTW; ++j) {
+          v4f z;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int co = 16 * (ct0 + j) + 4 * gq + q;
+            float a = b0[co];
+#pragma unroll
+            for (int tap = 0; tap < 9; ++tap) a += W0[co * 9 + tap] * sv[tap];
+            z[q] = gelu_t<GE>(a);
+          }
+          res[k][j] = v4f{spin, spin, spin, spin};
+          store4(k, j, z);
+        }
+      }
+    }
+  }
+  v8bf an[CTW][2];
+  load_a(1, 0, an);
+  __syncthreads();
+
+  for (int l = 1; l < 2 * nb; ++l) {
+    const float* bias = pf + bias_off + (int64_t)l * C;
+    v4f acc[MAXT][CTW];
+#pragma unroll
+    for (int j = 0; j < CTW; ++j) {
+      const int cb = 16 * (ct0 + j) + 4 * gq;
+      const v4f b4 = v4f{bias[cb], bias[cb + 1], bias[cb + 2], bias[cb + 3]};
+#pragma unroll
+      for (int k = 0; k < MAXT; ++k) acc[k][j] = b4;
+    }
+    const bool last = (l == 2 * nb - 1);
+    // the tap offsets are the same in every layer: an opaque per-layer zero
+    // keeps the compiler from hoisting all 9 x MAXT of them out of the layer
+    // loop into registers
+    int tz = 0;
+    asm volatile("" : "+s"(tz));
+#pragma unroll 1
+    for (int ks = 0; ks < 9; ++ks) {
+      v8bf a[CTW][2];
+#pragma unroll
+      for (int j = 0; j < CTW; ++j) {
+        a[j][0] = an[j][0];
+        a[j][1] = an[j][1];
+      }
+      if (ks < 8)
+        load_a(l, ks + 1, an);
+      else if (!last)
+        load_a(l + 1, 0, an);
+      const uint16_t* tk = tab + ks * RC32_MAXN + tz;
+      // tiles that exist for every wave run unconditionally (sites past N
+      // read the table's padding, offset 0, and are never stored); only the
+      // last is guarded
+#pragma unroll
+      for (int k = 0; k < MAXT; ++k) {
+        if (k < MAXT - 1 || k < nti) {
+          const uint32_t off = tk[site(k)];  // < RC32_MAXN for every tile read here
+          const v8bf bh = *reinterpret_cast<const v8bf*>(act + (off ^ xhi));
+          const v8bf bl = *reinterpret_cast<const v8bf*>(act + (off ^ xlo));
+#pragma unroll
+          for (int j = 0; j < CTW; ++j) {
+            acc[k][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[j][1], bh, acc[k][j], 0, 0, 0);
+            acc[k][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[j][0], bl, acc[k][j], 0, 0, 0);
+            acc[k][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[j][0], bh, acc[k][j], 0, 0, 0);
+          }
+        }
+      }
+    }
+    if (!last) {
+      const bool is_conv2 = (l % 2) == 1;
+      const float scale = is_conv2 ? rsqrtf((float)(l / 2 + 2)) : 1.0f;
+#pragma unroll
+      for (int k = 0; k < MAXT; ++k) {
+#pragma unroll
+        for (int j = 0; j < CTW; ++j) {
+          v4f o;
+          if (is_conv2) {
+            const v4f xr = acc[k][j] + res[k][j];
+            res[k][j] = xr;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) o[q] = gelu_t<GE>(xr[q] * scale);
+          } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) o[q] = gelu_t<GE>(acc[k][j][q]);
+          }
+          acc[k][j] = o;
+        }
+        // one tile's GELUs at a time: interleaving all of them costs the
+        // registers of a wave per SIMD
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      __syncthreads();  // every wave has read this layer's input
+#pragma unroll
+      for (int k = 0; k < MAXT; ++k)
+        if (k < nti)
+#pragma unroll
+          for (int j = 0; j < CTW; ++j) store4(k, j, acc[k][j]);
+      __syncthreads();
+    } else {
+      // head: h = (z2 + r) / sqrt(nb + 1); per-lane running logsumexp, one
+      // wave reduction, then the waves' (max, sum) merged in FP32
+      const float scale = rsqrtf((float)(nb + 1));
+      float m_l = -INFINITY, s_l = 0.0f;
+#pragma unroll
+      for (int k = 0; k < MAXT; ++k) {
+        if (k >= nti || site(k) >= N) continue;
+#pragma unroll
+        for (int j = 0; j < CTW; ++j) {
+          float hv[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) hv[q] = (acc[k][j][q] + res[k][j][q]) * scale;
+          const float mn = fmaxf(m_l, fmaxf(fmaxf(hv[0], hv[1]), fmaxf(hv[2], hv[3])));
+          float sacc = s_l * __expf(m_l - mn);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) sacc += __expf(hv[q] - mn);
+          s_l = sacc;
+          m_l = mn;
+        }
+      }
+      const float m_w = wave_max(m_l);
+      const float s_w = wave_sum(m_l > -INFINITY ? s_l * __expf(m_l - m_w) : 0.0f);
+      if (lane == 0) {
+        red[wv * 2] = m_w;
+        red[wv * 2 + 1] = s_w;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        float mx = -INFINITY;
+        for (int w = 0; w < W; ++w) mx = fmaxf(mx, red[w * 2]);
+        float ssum = 0.0f;
+        for (int w = 0; w < W; ++w)
+          if (red[w * 2] > -INFINITY) ssum += red[w * 2 + 1] * __expf(red[w * 2] - mx);
+        lp_out[0] = (double)mx + (double)logf(ssum) - g.log_cn;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// byte offsets (source site * 128 | (source site & 7) << 4) of the 9 taps of
+// every site, tab[tap * RC32_MAXN + p], padded with offset 0 up to
+// RC32_MAXN sites; all threads of the block call it
+__device__ __forceinline__ void rc32_build_tab(const RcGeom& g, uint16_t* tab) {
+  for (int e = threadIdx.x; e < 9 * RC32_MAXN; e += blockDim.x) {
+    const int tap = e / RC32_MAXN, p = e - tap * RC32_MAXN;
+    uint16_t v = 0;
+    if (p < g.N) {
+      const int src = tap_pos(g, p / g.L1, p % g.L1, tap);
+      v = (uint16_t)((src << 7) | ((src & 7) << 4));
+    }
+    tab[e] = v;
+  }
+}
+
 // ---------------------------------------------------------------- kernels
 struct RcArgs {
   RcGeom g;
@@ -1026,7 +1270,14 @@ struct RcArgs {
 
 #define RC_FWD(SP, LP)                                                                                        \
   do {                                                                                                        \
-    if constexpr (ROWS == 2)                                                                                  \
+    if constexpr (ROWS == 3) {                                                                                \
+      if (a.gelu_exact)                                                                                       \
+        rc_forward_t32<MAXT, W, RC32_CTW, true>(a.g, a.nb, a.packed, a.off_l0w, a.off_l0b, a.bias_off, a.off_frag,           \
+                                   a.frag_bytes, SP, act, reinterpret_cast<const uint16_t*>(wst), red, LP);   \
+      else                                                                                                    \
+        rc_forward_t32<MAXT, W, RC32_CTW, false>(a.g, a.nb, a.packed, a.off_l0w, a.off_l0b, a.bias_off, a.off_frag,          \
+                                    a.frag_bytes, SP, act, reinterpret_cast<const uint16_t*>(wst), red, LP);  \
+    } else if constexpr (ROWS == 2)                                                                           \
       rc_forward_win<MODE, B>(a.g, a.nb, a.gelu_exact, a.packed, a.off_l0w, a.off_l0b, a.bias_off, a.off_frag, \
                            a.frag_bytes, SP, act, wst, red, LP);                                              \
     else if constexpr (ROWS == 1)                                                                             \
@@ -1038,10 +1289,10 @@ struct RcArgs {
   } while (0)
 
 template <int C, int MODE, int B, int NMAX, int MAXT, int W, int ROWS>
-__global__ __launch_bounds__(W * 64, ((W == 8 || ROWS == 2) ? 2 : 1)) void rc_logpsi_kernel(RcArgs a, const int8_t* __restrict__ s, int64_t ns,
+__global__ __launch_bounds__(W * 64, (ROWS == 2 ? 3 : (ROWS == 3 ? 2 : (W == 8 ? 2 : 1)))) void rc_logpsi_kernel(RcArgs a, const int8_t* __restrict__ s, int64_t ns,
                                                                double* __restrict__ logabs) {
-  __shared__ __attribute__((aligned(16))) char act[2 * B * NMAX * C * 4];
-  __shared__ __attribute__((aligned(16))) char wst[ROWS == 2 ? rc_win_lds(B) : 16];
+  __shared__ __attribute__((aligned(16))) char act[(ROWS == 3 ? 1 : 2) * B * NMAX * C * 4];
+  __shared__ __attribute__((aligned(16))) char wst[ROWS == 2 ? rc_win_lds(B) : (ROWS == 3 ? rc32_tab_bytes(NMAX) : 16)];
   __shared__ int8_t sp[B * NMAX];
   __shared__ float red[2 * B * W];
   __shared__ double lp[B];
@@ -1051,18 +1302,19 @@ __global__ __launch_bounds__(W * 64, ((W == 8 || ROWS == 2) ? 2 : 1)) void rc_lo
     int b = i / N;
     sp[i] = (c0 + b < ns) ? s[(c0 + b) * N + (i % N)] : (int8_t)1;
   }
+  if constexpr (ROWS == 3) rc32_build_tab(a.g, reinterpret_cast<uint16_t*>(wst));
   __syncthreads();
   RC_FWD(sp, lp);
   if (threadIdx.x < B && c0 + threadIdx.x < ns) logabs[c0 + threadIdx.x] = lp[threadIdx.x];
 }
 
 template <int C, int MODE, int B, int NMAX, int MAXT, int W, int ROWS>
-__global__ __launch_bounds__(W * 64, ((W == 8 || ROWS == 2) ? 2 : 1)) void rc_metropolis_kernel(
+__global__ __launch_bounds__(W * 64, (ROWS == 2 ? 3 : (ROWS == 3 ? 2 : (W == 8 ? 2 : 1)))) void rc_metropolis_kernel(
     RcArgs a, int kind, const int32_t* __restrict__ nbr, int width, int hopping, const uint32_t* __restrict__ words,
     int nsteps, int64_t ns, double reweight, double tie_tol, int8_t* __restrict__ s, double* __restrict__ logabs,
     int32_t* __restrict__ n_accept, int32_t* __restrict__ n_near) {
-  __shared__ __attribute__((aligned(16))) char act[2 * B * NMAX * C * 4];
-  __shared__ __attribute__((aligned(16))) char wst[ROWS == 2 ? rc_win_lds(B) : 16];
+  __shared__ __attribute__((aligned(16))) char act[(ROWS == 3 ? 1 : 2) * B * NMAX * C * 4];
+  __shared__ __attribute__((aligned(16))) char wst[ROWS == 2 ? rc_win_lds(B) : (ROWS == 3 ? rc32_tab_bytes(NMAX) : 16)];
   __shared__ int8_t cur[B * NMAX];
   __shared__ int8_t prop[B * NMAX];
   __shared__ float red[2 * B * W];
@@ -1083,6 +1335,7 @@ __global__ __launch_bounds__(W * 64, ((W == 8 || ROWS == 2) ? 2 : 1)) void rc_me
     nacc[b] = 0;
     nnear[b] = 0;
   }
+  if constexpr (ROWS == 3) rc32_build_tab(a.g, reinterpret_cast<uint16_t*>(wst));
   __syncthreads();
   for (int b = wv; b < B; b += W) {
     int cnt = 0;
@@ -1181,11 +1434,11 @@ constexpr int RC_MAXCAND = 4096;
 constexpr int RC_MAXCAND_WIN = 3072;  // 16 x 16 window kernels: keeps two workgroups' LDS within a CU
 
 template <int C, int MODE, int B, int NMAX, int MAXT, int W, int ROWS>
-__global__ __launch_bounds__(W * 64, ((W == 8 || ROWS == 2) ? 2 : 1)) void rc_eloc_kernel(RcArgs a, qtx_ham H, const int8_t* __restrict__ s,
+__global__ __launch_bounds__(W * 64, (ROWS == 2 ? 3 : (ROWS == 3 ? 2 : (W == 8 ? 2 : 1)))) void rc_eloc_kernel(RcArgs a, qtx_ham H, const int8_t* __restrict__ s,
                                                              const double* __restrict__ logabs, int64_t ns,
                                                              double* __restrict__ eloc, int32_t* __restrict__ nconn) {
-  __shared__ __attribute__((aligned(16))) char act[2 * B * NMAX * C * 4];
-  __shared__ __attribute__((aligned(16))) char wst[ROWS == 2 ? rc_win_lds(B) : 16];
+  __shared__ __attribute__((aligned(16))) char act[(ROWS == 3 ? 1 : 2) * B * NMAX * C * 4];
+  __shared__ __attribute__((aligned(16))) char wst[ROWS == 2 ? rc_win_lds(B) : (ROWS == 3 ? rc32_tab_bytes(NMAX) : 16)];
   __shared__ int8_t s0[NMAX];
   __shared__ int8_t prop[B * NMAX];
   __shared__ float red[2 * B * W];
@@ -1197,6 +1450,7 @@ __global__ __launch_bounds__(W * 64, ((W == 8 || ROWS == 2) ? 2 : 1)) void rc_el
   const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
   const int64_t n = blockIdx.x;
   for (int i = threadIdx.x; i < N; i += (W * WAVE)) s0[i] = s[n * N + i];
+  if constexpr (ROWS == 3) rc32_build_tab(a.g, reinterpret_cast<uint16_t*>(wst));
   __syncthreads();
   if (wv == 0) {
     // diagonal part and the ordered list of valid candidates (ballot compaction)
@@ -1616,7 +1870,7 @@ static RcArgs rc_args(const RcLayout& L, const void* packed) {
       if (L.mode == QTX_CONV_FP32)                                                                     \
         hipLaunchKernelGGL((KERNEL<32, QTX_CONV_FP32, 1, 400, 7, 4, 0>), grid_, dim3(256), 0, st, __VA_ARGS__); \
       else                                                                                             \
-        hipLaunchKernelGGL((KERNEL<32, QTX_CONV_BF16X3, 1, 400, 7, 4, 0>), grid_, dim3(256), 0, st, __VA_ARGS__); \
+        hipLaunchKernelGGL((KERNEL<32, QTX_CONV_BF16X3, 1, RC32_MAXN, 7, 4, 3>), grid_, dim3(256), 0, st, __VA_ARGS__); \
     }                                                                                                  \
   } while (0)
 

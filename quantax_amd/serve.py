@@ -292,6 +292,11 @@ class Server:
         self.keep_results = keep_results
         self.progress = progress
         self._ws = None
+        # per round: host seconds spent enqueueing the grouped launches and the
+        # per-job pre/post work, host seconds inside qtx_minsr_solve_many
+        # (enqueue + its readbacks), round wall seconds, and HIP events around
+        # the three grouped launches
+        self.round_log = []
 
     def close(self) -> None:
         self._ws = None
@@ -313,12 +318,16 @@ class Server:
         """One VMC step of every active job: grouped sampling, E_loc and
         Jacobian on the current stream, then every job's solve on its own
         stream through one qtx_minsr_solve_many call."""
+        t0 = time.perf_counter()
+        ga, gb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ga.record()
         nst = [j.sweep_steps() for j in active]
         self._grouped("qtx_rbm_group_metropolis", active, nst)
         for j, n in zip(active, nst):
             j.advance_counters(n)
         self._grouped("qtx_rbm_group_eloc", active)
         self._grouped("qtx_rbm_group_jacobian", active)
+        gb.record()
         ready = torch.cuda.Event()
         ready.record()
         main = torch.cuda.current_stream()
@@ -327,7 +336,9 @@ class Server:
                 j.stream.wait_event(ready)
                 j.sr_pre()
         tasks = (_lib.QtxMinsrTask * len(active))(*[j.sr_task() for j in active])
+        t1 = time.perf_counter()
         rc = _lib.lib().qtx_minsr_solve_many(tasks, len(active))
+        t2 = time.perf_counter()
         if rc != 0:
             raise _lib.QtxError(f"qtx_minsr_solve_many failed ({rc}): "
                                 f"{_lib.lib().qtx_last_error().decode(errors='replace')}")
@@ -340,6 +351,23 @@ class Server:
                 j.done_event.record(j.stream)
         for j in active:
             main.wait_event(j.done_event)
+        t3 = time.perf_counter()
+        self.round_log.append(((t1 - t0) + (t3 - t2), t2 - t1, t3 - t0, ga, gb, len(active)))
+
+    def round_summary(self) -> dict:
+        """Mean per-round host and device times of the rounds run so far."""
+        if not self.round_log:
+            return {}
+        torch.cuda.synchronize()
+        enq = np.mean([r[0] for r in self.round_log])
+        solve = np.mean([r[1] for r in self.round_log])
+        wall = np.mean([r[2] for r in self.round_log])
+        grouped = np.mean([r[3].elapsed_time(r[4]) for r in self.round_log])
+        return {"rounds": len(self.round_log), "mean_jobs_per_round": round(float(np.mean([r[5] for r in self.round_log])), 2),
+                "round_wall_ms": round(1e3 * wall, 2),
+                "host_enqueue_ms": round(1e3 * enq, 2),
+                "solve_many_host_ms": round(1e3 * solve, 2),
+                "grouped_launches_gpu_ms": round(grouped, 2)}
 
     def run(self, specs: Sequence[JobSpec]) -> ServeReport:
         rep = ServeReport()

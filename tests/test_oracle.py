@@ -393,3 +393,23 @@ def test_update_skips_nonfinite():
     th = np.ones(4, np.float32)
     assert osr.update(th, np.array([1, np.nan, 0, 0])) is None
     assert np.array_equal(osr.update(th, np.array([0.5, 0, 0, 1.0])), np.array([0.5, 1, 1, 0], np.float32))
+
+
+@pytest.mark.parametrize("shape,nb,C", [((6, 6), 3, 16), ((5, 4), 2, 32)])
+def test_cpu_resconv_f32_matches_float64_restatement(shape, nb, C, rng):
+    """The FP32 torch-CPU forward (the CPU baseline's) against the float64
+    restatement, which the torch float64 autograd check above pins: agreement
+    to FP32 rounding; a dropped block, a missing bias or an unflipped
+    circular shift would move log psi by far more."""
+    from oracle import cpu_resconv
+
+    spec = om.ResConvSpec(nb, C, shape)
+    theta = spec.he_normal(rng).astype(np.float64)
+    for (cin, cout, hb), (w, bo) in zip(spec.layers, spec.offsets):
+        if hb:
+            theta[bo: bo + cout] = 0.1 * rng.standard_normal(cout)
+    theta = theta.astype(np.float32).astype(np.float64)
+    s = rand_spins(rng, 12, shape[0] * shape[1])
+    ref = om.resconv_forward(spec, theta, s)[1]
+    got = cpu_resconv.resconv_forward_f32(spec, theta, s)
+    assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)) < 2e-5
