@@ -47,6 +47,12 @@ extern "C" {
 #define QTX_SOLVE_CHOL_SHIFT 0        /* (T + shift I) y = b by Cholesky                */
 #define QTX_SOLVE_EIGH_PINV 1         /* soft pseudo-inverse of T, fastest exact route  */
 #define QTX_SOLVE_EIGH_PINV_DIRECT 2  /* the same soft pseudo-inverse, always by dsyevd */
+/* flags OR'd into a solver mode: the branch of the minnorm_* (T = Obar Obar^T)
+ * or lstsq_* (S = Obar^T Obar) factories whatever the shape (default: MinSR
+ * when ns < np, else SR, as auto_*), and the workspace for SNR truncation */
+#define QTX_SOLVE_MINNORM 0x100
+#define QTX_SOLVE_LSTSQ 0x200
+#define QTX_SOLVE_SNR 0x400
 
 /* ResConv forward precision */
 #define QTX_CONV_FP32 0   /* FP32 products on FP32 MFMA */
@@ -306,16 +312,40 @@ typedef struct qtx_minsr_task {
   size_t ws_bytes;
   int32_t path;         /* out: as qtx_minsr_last_path */
   int32_t fallback;     /* out: as qtx_minsr_last_path */
+  double tol_snr;       /* SNR truncation of the eigen-components (EIGH_PINV modes; > 1e-6 takes dsyevd and
+                           needs QTX_SOLVE_SNR in the mode of qtx_minsr_ws_bytes), 0: none */
 } qtx_minsr_task;
 int qtx_minsr_solve_many(qtx_minsr_task* tasks, int32_t ntasks);
+
+/* qtx_minsr_solve with the signal-to-noise truncation of minnorm_pinv_eig /
+ * lstsq_pinv_eig (optimizer/solver.py:114-164, _sum_without_noise): each
+ * eigen-component rho_s = sum_t c_ts of U^T b is divided by
+ * 1 + (tol_snr / snr_s)^6, snr_s = |mean_t c_ts| / sqrt(var_t c_ts / n_t), with
+ * c_ts = U_ts Ebar_t (MinSR; t over samples) or (Obar V)_ts Ebar_t (SR).
+ * mode: QTX_SOLVE_EIGH_PINV or _DIRECT, with optional branch flags; ws:
+ * qtx_minsr_ws_bytes(ns, np, mode | QTX_SOLVE_SNR, gram) bytes.  tol_snr <= 1e-6
+ * is no truncation, as in the reference. */
+int qtx_minsr_solve_snr(void* stream, int o_dtype, const void* O, const double* mean, const double* r,
+                        const double* ebar, int64_t ns, int64_t np, int mode, int gram, double rtol, double atol,
+                        double tol_snr, double* step, void* ws, size_t ws_bytes);
+
+/* minsr_pinv_eig (optimizer/solver.py:262-294): x = U g(D) rho for a given
+ * symmetric T (n x n, FP64, either triangle layout: T must be symmetric),
+ * T = U D U^T by rocSOLVER dsyevd, g the soft inverse with rtol, atol, and rho
+ * = U^T b with the SNR truncation above (c_ts = U_ts b_t) when tol_snr > 1e-6.
+ * ws: qtx_pinv_eig_ws_bytes(n) bytes. */
+size_t qtx_pinv_eig_ws_bytes(int64_t n);
+int qtx_pinv_eig(void* stream, const double* T, int64_t n, const double* b, double rtol, double atol, double tol_snr,
+                 double* x, void* ws, size_t ws_bytes);
 
 /* What the calling thread's last qtx_minsr_solve did: path 0 Cholesky shift,
  * 1 partial fractions, 2 dsyevd (-1: none or failed); Lanczos steps and the
  * lambda_max they gave (0 when not run); fallback (nullable): why EIGH_PINV
  * left the partial-fraction route for dsyevd: 0 it did not, 1 cutoff below
  * 1e-5 lambda_max, 2 non-finite Lanczos coefficients (non-finite T), 3
- * lambda_max <= 0, 4 a Cholesky factorisation failed.  Diagnostic only: no
- * counterpart in the reference, which always runs eigh. */
+ * lambda_max <= 0, 4 a Cholesky factorisation failed, 5 SNR truncation (needs
+ * the eigenvectors).  Diagnostic only: no counterpart in the reference, which
+ * always runs eigh. */
 int qtx_minsr_last_path(int* path, int* lanczos_steps, double* lambda_max, int* fallback);
 
 /* theta <- theta - lr * step in dtype, skipped entirely when any step entry is

@@ -1,8 +1,9 @@
 """SR / MinSR step in float64 (oracle).
 
 Restates ``quantax/optimizer/sr.py:74-195`` (Obar, Ebar, energy, VarE, solve)
-and ``quantax/optimizer/solver.py:12-201`` (shifted Cholesky and soft
-pseudo-inverse solvers, automatic SR/MinSR choice), plus the parameter update
+and ``quantax/optimizer/solver.py:12-201, 262-294`` (shifted Cholesky and soft
+pseudo-inverse solvers with the SNR truncation, automatic SR/MinSR choice,
+the pseudo-inverse of a given T), plus the parameter update
 ``quantax/state/variational.py:558-579`` (skip on non-finite, theta -= step).
 """
 from __future__ import annotations
@@ -69,29 +70,57 @@ def eigs_inv(vals, rtol=None, atol=0.0):
     return np.where(va > 0.0, inv, 0.0)
 
 
-def minnorm_pinv_eig(A, b, rtol=None, atol=0.0):
-    """solver.py:128-149 (tol_snr = 0): x = A^T U diag(eig_inv) U^T b."""
+def sum_without_noise(c, tol_snr):
+    """``_sum_without_noise`` (solver.py:114-125): the column sums of c (m, n)
+    divided by 1 + (tol_snr / snr)^6, snr = |mean| / sqrt(mean |c - mean|^2 / m),
+    applied only when tol_snr > 1e-6.  A zero sum stays zero (the reference's
+    expression is 0 / (1 + (tol / (0/0))^6) = NaN for an all-zero column)."""
+    c = np.asarray(c, np.float64)
+    m = c.shape[0]
+    x = np.sum(c, axis=0)
+    if not tol_snr > 1e-6:
+        return x
+    mean = x / m
+    sd = np.sqrt(np.mean(np.abs(c - mean[None, :]) ** 2, axis=0) / m)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        snr = np.abs(mean) / sd
+        out = x / (1 + (tol_snr / snr) ** 6)
+    return np.where(x == 0.0, 0.0, out)
+
+
+def minnorm_pinv_eig(A, b, rtol=None, atol=0.0, tol_snr=0.0):
+    """solver.py:128-149: x = A^T U diag(eig_inv) rho, rho = SNR-truncated
+    sums of c_ts = U_ts b_t."""
     T = A @ A.T
     vals, U = np.linalg.eigh(T)
     inv = eigs_inv(vals, rtol, atol)
-    rho = U.T @ b
+    rho = sum_without_noise(U * b[:, None], tol_snr)
     return A.T @ (U @ (inv * rho))
 
 
-def lstsq_pinv_eig(A, b, rtol=None, atol=0.0):
-    """solver.py:152-164 (tol_snr = 0): x = V diag(eig_inv) V^T A^T b."""
+def lstsq_pinv_eig(A, b, rtol=None, atol=0.0, tol_snr=0.0):
+    """solver.py:152-164: x = V diag(eig_inv) rho, rho = SNR-truncated sums of
+    c_sk = (A V)_sk b_s."""
     S = A.T @ A
     vals, V = np.linalg.eigh(S)
     inv = eigs_inv(vals, rtol, atol)
-    rho = V.T @ (A.T @ b)
+    rho = sum_without_noise((A @ V) * b[:, None], tol_snr)
     return V @ (inv * rho)
 
 
-def auto_pinv_eig(A, b, rtol=None, atol=0.0):
+def auto_pinv_eig(A, b, rtol=None, atol=0.0, tol_snr=0.0):
     """solver.py:167-201."""
     if A.shape[0] < A.shape[1]:
-        return minnorm_pinv_eig(A, b, rtol, atol)
-    return lstsq_pinv_eig(A, b, rtol, atol)
+        return minnorm_pinv_eig(A, b, rtol, atol, tol_snr)
+    return lstsq_pinv_eig(A, b, rtol, atol, tol_snr)
+
+
+def minsr_pinv_eig(T, b, rtol=None, atol=0.0, tol_snr=0.0):
+    """solver.py:262-294: x = U diag(eig_inv) rho for a given symmetric T."""
+    vals, U = np.linalg.eigh(np.asarray(T, np.float64))
+    inv = eigs_inv(vals, rtol, atol)
+    rho = sum_without_noise(U * np.asarray(b, np.float64)[:, None], tol_snr)
+    return U @ (inv * rho)
 
 
 def sr_step(O, eloc, solver, r=None):

@@ -13,10 +13,12 @@ import torch  # noqa: F401  (loads the HIP runtime before the library)
 
 from .build import LIB
 
+QTX_OK, QTX_EINVAL, QTX_EHIP, QTX_ESOLVER, QTX_EWORKSPACE = 0, -1, -2, -3, -4
 QTX_F32, QTX_F64 = 0, 1
 QTX_LOCALFLIP, QTX_EXCHANGE = 0, 1
 QTX_GRAM_FP64, QTX_GRAM_FP32, QTX_GRAM_BF16X3 = 0, 1, 2
 QTX_SOLVE_CHOL_SHIFT, QTX_SOLVE_EIGH_PINV, QTX_SOLVE_EIGH_PINV_DIRECT = 0, 1, 2
+QTX_SOLVE_MINNORM, QTX_SOLVE_LSTSQ, QTX_SOLVE_SNR = 0x100, 0x200, 0x400
 QTX_CONV_FP32, QTX_CONV_BF16X3 = 0, 1
 TAG_SWEEP, TAG_INIT = 0x51A7, 0x1D17
 
@@ -43,6 +45,7 @@ class QtxMinsrTask(C.Structure):
         ("stream", vp), ("o_dtype", i32), ("mode", i32), ("gram", i32), ("status", i32), ("O", vp), ("mean", vp),
         ("r", vp), ("ebar", vp), ("ns", i64), ("np", i64), ("rshift", f64), ("ashift", f64), ("rtol", f64),
         ("atol", f64), ("step", vp), ("ws", vp), ("ws_bytes", sz), ("path", i32), ("fallback", i32),
+        ("tol_snr", f64),
     ]
 
 
@@ -95,6 +98,10 @@ PROTOS = {
     "qtx_minsr_solve": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, i64, i64, C.c_int, C.c_int, f64, f64, f64, f64, vp,
                                   vp, sz]),
     "qtx_minsr_solve_many": (C.c_int, [C.POINTER(QtxMinsrTask), i32]),
+    "qtx_minsr_solve_snr": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, i64, i64, C.c_int, C.c_int, f64, f64, f64, vp,
+                                      vp, sz]),
+    "qtx_pinv_eig_ws_bytes": (sz, [i64]),
+    "qtx_pinv_eig": (C.c_int, [vp, vp, i64, vp, f64, f64, f64, vp, vp, sz]),
     "qtx_gram_ws_bytes": (sz, [i64, i64, C.c_int]),
     "qtx_gram": (C.c_int, [vp, C.c_int, vp, vp, vp, i64, i64, C.c_int, vp, vp, sz]),
     "qtx_minsr_last_path": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(f64), C.POINTER(C.c_int)]),

@@ -737,6 +737,59 @@ __global__ void refine_accum_kernel(const double* __restrict__ Z, int64_t n, int
   x[i] = first ? v : x[i] + v;
 }
 
+// ---------------------------------------------------------------- SNR truncation
+// _sum_without_noise (optimizer/solver.py:114-125): rho_s = sum_t c_ts is
+// divided by 1 + (tol / snr_s)^6, snr_s = |mean_s| / sqrt(var_s / m), with the
+// variance about the mean as the reference forms it (two passes: rho is known
+// before the sums of squares are taken).
+// var2[s] = sum_t (U[t, s] y[t] - rho[s] / m)^2, U column-major n x n, one
+// block per column (MinSR and minsr_pinv_eig: c_ts = U_ts b_t)
+__global__ __launch_bounds__(256) void snr_colsq_kernel(const double* __restrict__ U, int64_t n,
+                                                        const double* __restrict__ y, const double* __restrict__ rho,
+                                                        double m, double* __restrict__ var2) {
+  __shared__ double red[4];
+  const int64_t s = blockIdx.x;
+  const double* col = U + s * n;
+  const double mu = rho[s] / m;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double v = col[i] * y[i] - mu;
+    acc += v * v;
+  }
+  acc = wave_sum(acc);
+  if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) var2[s] = red[0] + red[1] + red[2] + red[3];
+}
+
+// var2[k] (+)= sum_s (C[k + s np] e[s] - rho[k] / m)^2 over a block of rows s
+// (C = Obar V of the block, column-major np x rows: SR, c_sk = (Obar V)_sk Ebar_s)
+__global__ __launch_bounds__(256) void snr_rowsq_kernel(const double* __restrict__ Cb, int64_t np, int64_t rows,
+                                                        const double* __restrict__ e, const double* __restrict__ rho,
+                                                        double m, int first, double* __restrict__ var2) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  const double mu = rho[k] / m;
+  double acc = 0.0;
+  for (int64_t s = 0; s < rows; ++s) {
+    const double v = Cb[k + s * np] * e[s] - mu;
+    acc += v * v;
+  }
+  var2[k] = (first ? 0.0 : var2[k]) + acc;
+}
+
+__global__ void snr_scale_kernel(double* __restrict__ rho, const double* __restrict__ var2, int64_t n, double m,
+                                 double tol) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  // rho = 0 stays 0 (the reference's 0 / (1 + (tol / (0/0))^6) is NaN there)
+  if (rho[i] == 0.0) return;
+  const double mean = rho[i] / m;
+  const double snr = fabs(mean) / sqrt(var2[i] / m / m);
+  const double q = tol / snr, q2 = q * q;
+  rho[i] = rho[i] / (1.0 + q2 * q2 * q2);
+}
+
 // ---------------------------------------------------------------- rocBLAS handle
 // One handle per (thread, device, stream).  A handle carries the device
 // workspace rocSOLVER uses for its temporaries, so one handle switched between
@@ -803,6 +856,16 @@ static int64_t obar_rows(int64_t ns, int64_t np) {
   return std::max<int64_t>(1, std::min<int64_t>(ns, (int64_t(1) << 28) / (8 * std::max<int64_t>(np, 1))));
 }
 
+// solver mode without its flags, and the branch: T = Obar Obar^T (MinSR) or
+// S = Obar^T Obar (SR), by the flags or, without them, by the shape
+static inline int base_mode(int mode) { return mode & 0xff; }
+static inline bool use_minnorm(int64_t ns, int64_t np, int mode) {
+  if (mode & QTX_SOLVE_MINNORM) return true;
+  if (mode & QTX_SOLVE_LSTSQ) return false;
+  return ns < np;
+}
+constexpr double kSnrMin = 1e-6;  // tol_snr at or below it: no truncation (solver.py:124)
+
 struct SolveWs {
   double* T;       // n x n
   double* y;       // max(ns, np)
@@ -824,10 +887,13 @@ struct SolveWs {
   double* ab;      // alpha[kLanczosMax], beta[kLanczosMax + 1]
   void* split;     // bf16 hi / lo planes of Obar (split-bf16 Gram only)
   double* obar;    // ns x np FP64 Obar (SR branch, ns >= np: S = Obar^T Obar by DGEMM)
+  double* var2;    // n: SNR truncation, sums of squared deviations (QTX_SOLVE_SNR)
+  double* cbuf;    // obar_rows x np: Obar V of a row block (SR branch with QTX_SOLVE_SNR)
 };
 
 static size_t ws_layout(int64_t ns, int64_t np, int mode, int gram, SolveWs* w, char* base) {
-  int64_t n = ns < np ? ns : np;
+  const bool mn = use_minnorm(ns, np, mode);
+  int64_t n = mn ? ns : np;
   int64_t m = ns > np ? ns : np;
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -844,7 +910,7 @@ static size_t ws_layout(int64_t ns, int64_t np, int mode, int gram, SolveWs* w, 
   s.scalars = (double*)take(64);
   s.info = (int32_t*)take(3 * kInfoPerFactor * sizeof(int32_t));
   s.dtile = (double*)take((size_t)3 * 64 * 64 * 8);
-  if (mode == QTX_SOLVE_EIGH_PINV) {
+  if (base_mode(mode) == QTX_SOLVE_EIGH_PINV) {
     s.T2 = (double*)take((size_t)3 * n * n * 8);
     s.R = (double*)take((size_t)n * 48);
     s.TW = (double*)take((size_t)n * 48);
@@ -854,8 +920,12 @@ static size_t ws_layout(int64_t ns, int64_t np, int mode, int gram, SolveWs* w, 
     s.w = (double*)take((size_t)n * 8);
     s.ab = (double*)take((size_t)(2 * kLanczosMax + 1) * 8);
   }
-  if (gram == QTX_GRAM_BF16X3 && ns < np) s.split = take(g3_bytes(ns, np));
-  if (ns >= np) s.obar = (double*)take((size_t)obar_rows(ns, np) * np * 8);
+  if (gram == QTX_GRAM_BF16X3 && mn) s.split = take(g3_bytes(ns, np));
+  if (!mn) s.obar = (double*)take((size_t)obar_rows(ns, np) * np * 8);
+  if (mode & QTX_SOLVE_SNR) {
+    s.var2 = (double*)take((size_t)n * 8);
+    if (!mn) s.cbuf = (double*)take((size_t)obar_rows(ns, np) * np * 8);
+  }
   if (w) *w = s;
   return off;
 }
@@ -936,8 +1006,48 @@ static int launch_obar_t_y(hipStream_t st, const TI* O, const double* mean, cons
   return QTX_OK;
 }
 
-// y <- U g(D) U^T y through rocSOLVER dsyevd (T is overwritten); returns 1 or an error
-static int pinv_syevd(hipStream_t st, rocblas_handle h, int64_t n, double rtol, double atol, SolveWs& w) {
+// What the SNR truncation of the SR branch needs besides the eigenvectors:
+// the per-sample contributions are (Obar V)_sk Ebar_s, Obar formed again a
+// row block at a time
+struct SnrRows {
+  int o_dtype;
+  const void* O;
+  const double* mean;
+  const double* r;
+  const double* ebar;
+  int64_t ns, np;
+};
+
+template <typename TI>
+static int snr_rows_var2(hipStream_t st, rocblas_handle h, const SnrRows& L, SolveWs& w) {
+  const int64_t rows = obar_rows(L.ns, L.np);
+  const double one = 1.0, zero = 0.0;
+  const TI* O = (const TI*)L.O;
+  for (int64_t s0 = 0; s0 < L.ns; s0 += rows) {
+    const int64_t nr = std::min(rows, L.ns - s0), tot = nr * L.np;
+    hipLaunchKernelGGL(form_obar_kernel<TI>, dim3((unsigned)std::min<int64_t>((tot + 255) / 256, kGridCap)), dim3(256),
+                       0, st, O + s0 * L.np, L.mean, L.r ? L.r + s0 : nullptr, nr, L.np, 1.0 / (double)L.ns, w.obar);
+    QTX_LAUNCH_CHECK();
+    // (Obar V)^T of the block, column-major np x nr: V^T B with B the block
+    // as a column-major np x nr matrix
+    int rc = rb(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)L.np,
+                              (rocblas_int)nr, (rocblas_int)L.np, &one, w.T, (rocblas_int)L.np, w.obar,
+                              (rocblas_int)L.np, &zero, w.cbuf, (rocblas_int)L.np),
+                "rocblas_dgemm");
+    if (rc) return rc;
+    hipLaunchKernelGGL(snr_rowsq_kernel, dim3(nblk(L.np, 256)), dim3(256), 0, st, w.cbuf, L.np, nr, L.ebar + s0, w.z,
+                       (double)L.ns, s0 == 0 ? 1 : 0, w.var2);
+    QTX_LAUNCH_CHECK();
+  }
+  return QTX_OK;
+}
+
+// y <- U g(D) U^T y through rocSOLVER dsyevd (T is overwritten); with
+// tol_snr > 1e-6 the components of U^T y are truncated by their SNR (rows ==
+// nullptr: c_ts = U_ts y_t over the n rows of T; else the SR branch's
+// (Obar V)_sk Ebar_s over the samples); returns 1 or an error
+static int pinv_syevd(hipStream_t st, rocblas_handle h, int64_t n, double rtol, double atol, SolveWs& w,
+                      double tol_snr = 0.0, const SnrRows* rows = nullptr) {
   const rocblas_int ni = (rocblas_int)n;
   // exact power-of-two rescaling of T (eigenvectors unchanged, eigenvalues
   // scaled by 2^k and undone in eig_inv_kernel)
@@ -954,6 +1064,21 @@ static int pinv_syevd(hipStream_t st, rocblas_handle h, int64_t n, double rtol, 
   rc = rb(rocblas_dgemv(h, rocblas_operation_transpose, ni, ni, &one, w.T, ni, w.y, 1, &zero, w.z, 1),
           "rocblas_dgemv");
   if (rc) return rc;
+  if (tol_snr > kSnrMin) {
+    QTX_REQUIRE(w.var2 != nullptr, "tol_snr > 0 needs the QTX_SOLVE_SNR workspace");
+    double m = (double)n;
+    if (rows == nullptr) {
+      hipLaunchKernelGGL(snr_colsq_kernel, dim3((unsigned)n), dim3(256), 0, st, w.T, n, w.y, w.z, m, w.var2);
+      QTX_LAUNCH_CHECK();
+    } else {
+      QTX_REQUIRE(w.cbuf != nullptr, "tol_snr > 0 needs the QTX_SOLVE_SNR workspace");
+      rc = rows->o_dtype == QTX_F32 ? snr_rows_var2<float>(st, h, *rows, w) : snr_rows_var2<double>(st, h, *rows, w);
+      if (rc) return rc;
+      m = (double)rows->ns;
+    }
+    hipLaunchKernelGGL(snr_scale_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, w.z, w.var2, n, m, tol_snr);
+    QTX_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL(eig_inv_kernel, dim3(1), dim3(1024), 0, st, w.D, n, rtol, atol, w.scalars + 1, w.z);
   QTX_LAUNCH_CHECK();
   rc = rb(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &one, w.T, ni, w.z, 1, &zero, w.y, 1), "rocblas_dgemv");
@@ -1271,7 +1396,8 @@ static int phase1(Task& t, rocblas_handle h) {
     if (rc) return rc;
   }
   const rocblas_int ni = (rocblas_int)t.n;
-  if (io.mode == QTX_SOLVE_CHOL_SHIFT) {
+  const int mode = base_mode(io.mode);
+  if (mode == QTX_SOLVE_CHOL_SHIFT) {
     hipLaunchKernelGGL(shift_diag_kernel, dim3(1), dim3(1024), 0, t.st, t.w.T, t.n, io.rshift, io.ashift,
                        t.w.scalars);
     QTX_LAUNCH_CHECK();
@@ -1281,7 +1407,12 @@ static int phase1(Task& t, rocblas_handle h) {
     if (rc) return rc;
     return rb(rocsolver_dpotrs(h, rocblas_fill_lower, ni, 1, t.w.T, ni, t.w.y, ni), "rocsolver_dpotrs");
   }
-  if (io.mode == QTX_SOLVE_EIGH_PINV_DIRECT) {
+  if (mode == QTX_SOLVE_EIGH_PINV_DIRECT) {
+    t.need_syevd = true;
+    return QTX_OK;
+  }
+  if (io.tol_snr > kSnrMin) {  // the truncation needs the eigenvectors
+    t.fallback = 5;
     t.need_syevd = true;
     return QTX_OK;
   }
@@ -1301,9 +1432,15 @@ static int phase1(Task& t, rocblas_handle h) {
   return lanczos_enqueue(t, h, t.mcap < first ? t.mcap : first);
 }
 
+static int snr_syevd(Task& t, rocblas_handle h) {
+  qtx_minsr_task& io = *t.io;
+  SnrRows rows{io.o_dtype, io.O, io.mean, io.r, io.ebar, io.ns, io.np};
+  return pinv_syevd(t.st, h, t.n, io.rtol, io.atol, t.w, io.tol_snr, t.minnorm ? nullptr : &rows);
+}
+
 static int phase2(Task& t, rocblas_handle h) {
   qtx_minsr_task& io = *t.io;
-  if (io.mode == QTX_SOLVE_CHOL_SHIFT) return QTX_OK;
+  if (base_mode(io.mode) == QTX_SOLVE_CHOL_SHIFT) return QTX_OK;
   if (!t.need_syevd) {
     std::vector<double> ha(kLanczosMax), hb(kLanczosMax + 1);
     double lmax = 0.0;
@@ -1414,7 +1551,7 @@ static int phase2(Task& t, rocblas_handle h) {
     }
   }
   if (t.need_syevd) {
-    int rc = pinv_syevd(t.st, h, t.n, io.rtol, io.atol, t.w);
+    int rc = snr_syevd(t, h);
     if (rc < 0) return rc;
     t.path = 2;
   }
@@ -1424,22 +1561,23 @@ static int phase2(Task& t, rocblas_handle h) {
 template <typename TI>
 static int phase3(Task& t, rocblas_handle h) {
   qtx_minsr_task& io = *t.io;
-  if (io.mode == QTX_SOLVE_CHOL_SHIFT || t.pf) {
+  const bool chol = base_mode(io.mode) == QTX_SOLVE_CHOL_SHIFT;
+  if (chol || t.pf) {
     std::vector<int32_t> info_h(3 * kInfoPerFactor);
     QTX_HIP(hipMemcpyAsync(info_h.data(), t.w.info, info_h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, t.st));
     QTX_HIP(hipStreamSynchronize(t.st));
     int bad = -1;  // first diagonal block whose factorisation failed
-    const int nslots = (io.mode == QTX_SOLVE_CHOL_SHIFT) ? kInfoPerFactor : 3 * kInfoPerFactor;
+    const int nslots = chol ? kInfoPerFactor : 3 * kInfoPerFactor;
     for (int i = 0; i < nslots && bad < 0; ++i)
       if (info_h[i] != 0) bad = i;
-    if (io.mode == QTX_SOLVE_CHOL_SHIFT) {
+    if (chol) {
       if (bad >= 0)
         return fail(QTX_ESOLVER, "potrf failed in diagonal block " + std::to_string(bad) +
                                      " (info " + std::to_string(info_h[bad]) + ")");
       t.path = 0;
     } else if (bad >= 0) {  // T and y are intact: take the dsyevd route
       t.fallback = 4;
-      int rc = pinv_syevd(t.st, h, t.n, io.rtol, io.atol, t.w);
+      int rc = snr_syevd(t, h);
       if (rc < 0) return rc;
       t.path = 2;
     } else {
@@ -1461,12 +1599,17 @@ static int phase3(Task& t, rocblas_handle h) {
 static int check_task(const qtx_minsr_task& io) {
   QTX_REQUIRE(io.o_dtype == QTX_F32 || io.o_dtype == QTX_F64, "o_dtype must be QTX_F32 or QTX_F64");
   QTX_REQUIRE(io.ns > 0 && io.np > 0 && io.O && io.mean && io.ebar && io.step, "bad arguments");
-  QTX_REQUIRE(io.mode == QTX_SOLVE_CHOL_SHIFT || io.mode == QTX_SOLVE_EIGH_PINV ||
-                  io.mode == QTX_SOLVE_EIGH_PINV_DIRECT,
+  const int mode = base_mode(io.mode);
+  QTX_REQUIRE(mode == QTX_SOLVE_CHOL_SHIFT || mode == QTX_SOLVE_EIGH_PINV || mode == QTX_SOLVE_EIGH_PINV_DIRECT,
               "unknown solver mode");
+  QTX_REQUIRE((io.mode & ~(0xff | QTX_SOLVE_MINNORM | QTX_SOLVE_LSTSQ | QTX_SOLVE_SNR)) == 0 &&
+                  !((io.mode & QTX_SOLVE_MINNORM) && (io.mode & QTX_SOLVE_LSTSQ)),
+              "bad solver mode flags");
   QTX_REQUIRE(io.gram == QTX_GRAM_FP64 || io.gram == QTX_GRAM_FP32 || io.gram == QTX_GRAM_BF16X3,
               "unknown gram mode");
-  QTX_REQUIRE((io.ns < io.np ? io.ns : io.np) <= 131072, "matrix too large (n > 131072)");
+  QTX_REQUIRE((use_minnorm(io.ns, io.np, io.mode) ? io.ns : io.np) <= 131072, "matrix too large (n > 131072)");
+  QTX_REQUIRE(!(io.tol_snr > kSnrMin) || mode == QTX_SOLVE_CHOL_SHIFT || (io.mode & QTX_SOLVE_SNR),
+              "tol_snr > 0 needs QTX_SOLVE_SNR in the mode (and its workspace)");
   size_t need = ws_layout(io.ns, io.np, io.mode, io.gram, nullptr, nullptr);
   if (io.ws == nullptr || io.ws_bytes < need)
     return fail(QTX_EWORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
@@ -1506,7 +1649,7 @@ static int solve_many(qtx_minsr_task* tasks, int ntasks) {
       continue;
     }
     t.st = as_stream(tasks[i].stream);
-    t.minnorm = tasks[i].ns < tasks[i].np;
+    t.minnorm = use_minnorm(tasks[i].ns, tasks[i].np, tasks[i].mode);
     t.n = t.minnorm ? tasks[i].ns : tasks[i].np;
     ws_layout(tasks[i].ns, tasks[i].np, tasks[i].mode, tasks[i].gram, &t.w, (char*)tasks[i].ws);
   }
@@ -1524,7 +1667,7 @@ static int solve_many(qtx_minsr_task* tasks, int ntasks) {
         else if (phase == 2) rc = phase2(t, h);
         else rc = f32 ? phase3<float>(t, h) : phase3<double>(t, h);
         // factorisations of one task finish before the next task's start
-        const bool factored = (phase == 1 && t.io->mode == QTX_SOLVE_CHOL_SHIFT) || (phase == 2 && t.pf);
+        const bool factored = (phase == 1 && base_mode(t.io->mode) == QTX_SOLVE_CHOL_SHIFT) || (phase == 2 && t.pf);
         if (rc == QTX_OK && factored && ntasks > 1 && serial_factorisations())
           rc = hip_check(hipStreamSynchronize(t.st), "hipStreamSynchronize");
         rocblas_set_pointer_mode(h, pm);
@@ -1695,6 +1838,70 @@ int qtx_apply_update(void* stream, int dtype, void* theta, const double* step, d
     hipLaunchKernelGGL(apply_update_kernel<double>, dim3(nblk(np, 256)), dim3(256), 0, st, (double*)theta, step,
                        lr, np, applied);
   QTX_LAUNCH_CHECK();
+  return QTX_OK;
+}
+
+int qtx_minsr_solve_snr(void* stream, int o_dtype, const void* O, const double* mean, const double* r,
+                        const double* ebar, int64_t ns, int64_t np, int mode, int gram, double rtol, double atol,
+                        double tol_snr, double* step, void* ws, size_t ws_bytes) {
+  QTX_REQUIRE(base_mode(mode) == QTX_SOLVE_EIGH_PINV || base_mode(mode) == QTX_SOLVE_EIGH_PINV_DIRECT,
+              "SNR truncation applies to the EIGH_PINV modes");
+  QTX_REQUIRE(tol_snr >= 0.0, "tol_snr must be >= 0");
+  qtx_minsr_task task{};
+  task.stream = stream;
+  task.o_dtype = o_dtype;
+  task.mode = mode | QTX_SOLVE_SNR;
+  task.gram = gram;
+  task.O = O;
+  task.mean = mean;
+  task.r = r;
+  task.ebar = ebar;
+  task.ns = ns;
+  task.np = np;
+  task.rtol = rtol;
+  task.atol = atol;
+  task.tol_snr = tol_snr;
+  task.step = step;
+  task.ws = ws;
+  task.ws_bytes = ws_bytes;
+  g_last_path = -1;
+  g_last_lanczos = 0;
+  g_last_lmax = 0.0;
+  g_last_fallback = 0;
+  return solve_many(&task, 1);
+}
+
+// the MinSR-branch layout of an n x n problem: T, y, z, D, E, scalars and var2
+static constexpr int kPinvEigMode = QTX_SOLVE_EIGH_PINV_DIRECT | QTX_SOLVE_MINNORM | QTX_SOLVE_SNR;
+
+size_t qtx_pinv_eig_ws_bytes(int64_t n) {
+  if (n <= 0) return 0;
+  return ws_layout(n, n, kPinvEigMode, QTX_GRAM_FP64, nullptr, nullptr);
+}
+
+int qtx_pinv_eig(void* stream, const double* T, int64_t n, const double* b, double rtol, double atol, double tol_snr,
+                 double* x, void* ws, size_t ws_bytes) {
+  QTX_REQUIRE(n > 0 && T && b && x, "bad arguments");
+  QTX_REQUIRE(n <= 131072, "matrix too large (n > 131072)");
+  QTX_REQUIRE(rtol >= 0.0 && atol >= 0.0 && tol_snr >= 0.0, "tolerances must be >= 0");
+  const size_t need = qtx_pinv_eig_ws_bytes(n);
+  if (ws == nullptr || ws_bytes < need)
+    return fail(QTX_EWORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+  hipStream_t st = as_stream(stream);
+  SolveWs w{};
+  ws_layout(n, n, kPinvEigMode, QTX_GRAM_FP64, &w, (char*)ws);
+  QTX_HIP(hipMemcpyAsync(w.T, T, (size_t)n * n * 8, hipMemcpyDeviceToDevice, st));
+  QTX_HIP(hipMemcpyAsync(w.y, b, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
+  rocblas_handle h = nullptr;
+  int rc = get_blas(st, &h);
+  if (rc) return rc;
+  rocblas_pointer_mode pm;
+  rocblas_get_pointer_mode(h, &pm);
+  rocblas_set_pointer_mode(h, rocblas_pointer_mode_host);
+  rc = pinv_syevd(st, h, n, rtol, atol, w, tol_snr, nullptr);
+  rocblas_set_pointer_mode(h, pm);
+  if (rc < 0) return rc;
+  QTX_HIP(hipMemcpyAsync(x, w.y, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
   return QTX_OK;
 }
 

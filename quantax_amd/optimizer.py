@@ -4,10 +4,13 @@ Mirrors ``quantax/optimizer/sr.py:17-195`` (QNGD/SR: Ebar, Obar, solve,
 get_step, energy, VarE), the solver factories of
 ``quantax/optimizer/solver.py:50-201`` and ``Euler`` of
 ``quantax/optimizer/driver.py:30-39``.  A solver here is a small descriptor
-(mode, shifts, tolerances, Gram precision) executed by ``qtx_minsr_solve``:
-Ns < Np selects MinSR (T = Obar Obar^T, Ns x Ns), else SR (S = Obar^T Obar),
-exactly like the ``auto_*`` factories.  Obar is formed on the fly on the
-device; nothing of size Ns x Np is materialised beyond O itself.  The
+(mode, shifts, tolerances, SNR tolerance, Gram precision, branch) executed by
+``qtx_minsr_solve`` / ``qtx_minsr_solve_snr``: the ``auto_*`` factories take
+MinSR (T = Obar Obar^T, Ns x Ns) when Ns < Np, else SR (S = Obar^T Obar); the
+``minnorm_*`` / ``lstsq_*`` factories take their branch whatever the shape.
+``minsr_pinv_eig`` solves T x = b for a given T (``solver.py:262-294``).
+Obar is formed on the fly on the device; nothing of size Ns x Np is
+materialised beyond O itself.  The
 momentum variants SPRING, MARCH and AdamSR (``sr.py:198-429``) reuse the same
 solve, with Obar v products and the column scaling of Obar done by
 ``qtx_obar_matvec`` and ``qtx_scale_columns``.
@@ -36,23 +39,47 @@ class Solver:
     atol: float = 0.0
     gram: str = "fp64"
     branch: str = "auto"  # "auto" | "minnorm" | "lstsq": the reference factory this came from
+    tol_snr: float = 0.0  # SNR truncation of the eigen-components (pinv solvers), <= 1e-6: none
 
-    def check(self, ns: int, np_: int) -> None:
-        if self.branch == "minnorm" and not ns < np_:
-            raise ValueError("minnorm_* solvers need nsamples < nparams here; use the auto_* factory")
-        if self.branch == "lstsq" and ns < np_:
-            raise ValueError("lstsq_* solvers need nsamples >= nparams here; use the auto_* factory")
+    @property
+    def snr(self) -> bool:
+        return self.tol_snr > 1e-6  # solver.py:124
+
+    @property
+    def cmode(self) -> int:
+        """The mode word of the C-ABI: the base mode, the branch flag of a
+        minnorm_* / lstsq_* factory and the SNR workspace flag."""
+        m = self.mode
+        if self.branch == "minnorm":
+            m |= _lib.QTX_SOLVE_MINNORM
+        elif self.branch == "lstsq":
+            m |= _lib.QTX_SOLVE_LSTSQ
+        if self.snr:
+            m |= _lib.QTX_SOLVE_SNR
+        return m
+
+    def ws_bytes(self, ns: int, np_: int) -> int:
+        return int(_lib.lib().qtx_minsr_ws_bytes(ns, np_, self.cmode, _GRAM[self.gram]))
 
 
 def _rt(v):
     return 1e-12 if v is None else float(v)  # _get_rtol for float64 (solver.py:12-21)
 
 
+def _snr(v) -> float:
+    v = float(v)
+    if not v >= 0.0:
+        raise ValueError("tol_snr must be >= 0")
+    return v
+
+
 def minnorm_shift_eig(rshift: Optional[float] = None, ashift: float = 1e-4, gram: str = "fp64") -> Solver:
+    """x = Obar^T (Obar Obar^T + shift)^{-1} Ebar whatever the shape (solver.py:50-62)."""
     return Solver(_lib.QTX_SOLVE_CHOL_SHIFT, rshift=_rt(rshift), ashift=ashift, gram=gram, branch="minnorm")
 
 
 def lstsq_shift_eig(rshift: Optional[float] = None, ashift: float = 1e-4, gram: str = "fp64") -> Solver:
+    """x = (Obar^T Obar + shift)^{-1} Obar^T Ebar whatever the shape (solver.py:65-77)."""
     return Solver(_lib.QTX_SOLVE_CHOL_SHIFT, rshift=_rt(rshift), ashift=ashift, gram=gram, branch="lstsq")
 
 
@@ -62,38 +89,65 @@ def auto_shift_eig(rshift: Optional[float] = None, ashift: float = 1e-4, gram: s
 
 def minnorm_pinv_eig(rtol: Optional[float] = None, atol: float = 0.0, tol_snr: float = 0.0,
                      gram: str = "fp64") -> Solver:
-    if tol_snr != 0.0:
-        raise NotImplementedError("tol_snr > 0 is not supported")
-    return Solver(_lib.QTX_SOLVE_EIGH_PINV, rtol=_rt(rtol), atol=atol, gram=gram, branch="minnorm")
+    """MinSR by the soft pseudo-inverse of T = Obar Obar^T (solver.py:128-149)."""
+    return Solver(_lib.QTX_SOLVE_EIGH_PINV, rtol=_rt(rtol), atol=atol, gram=gram, branch="minnorm",
+                  tol_snr=_snr(tol_snr))
 
 
 def lstsq_pinv_eig(rtol: Optional[float] = None, atol: float = 0.0, tol_snr: float = 0.0,
                    gram: str = "fp64") -> Solver:
-    if tol_snr != 0.0:
-        raise NotImplementedError("tol_snr > 0 is not supported")
-    return Solver(_lib.QTX_SOLVE_EIGH_PINV, rtol=_rt(rtol), atol=atol, gram=gram, branch="lstsq")
+    """SR by the soft pseudo-inverse of S = Obar^T Obar (solver.py:152-164)."""
+    return Solver(_lib.QTX_SOLVE_EIGH_PINV, rtol=_rt(rtol), atol=atol, gram=gram, branch="lstsq",
+                  tol_snr=_snr(tol_snr))
 
 
 def auto_pinv_eig(rtol: Optional[float] = None, atol: float = 0.0, tol_snr: float = 0.0,
                   gram: str = "fp64") -> Solver:
-    if tol_snr != 0.0:
-        raise NotImplementedError("tol_snr > 0 is not supported")
-    return Solver(_lib.QTX_SOLVE_EIGH_PINV, rtol=_rt(rtol), atol=atol, gram=gram)
+    """MinSR when Ns < Np, else SR (solver.py:167-201)."""
+    return Solver(_lib.QTX_SOLVE_EIGH_PINV, rtol=_rt(rtol), atol=atol, gram=gram, tol_snr=_snr(tol_snr))
+
+
+def minsr_pinv_eig(rtol: Optional[float] = None, atol: float = 0.0, tol_snr: float = 0.0):
+    """The solver of ``T x = b`` for a given symmetric T (solver.py:262-294):
+    x = U g(D) rho, T = U D U^T by rocSOLVER dsyevd, rho = U^T b with the SNR
+    truncation when tol_snr > 1e-6.  Returns ``solve(T, b) -> x`` on FP64
+    device tensors."""
+    rt, at, ts = _rt(rtol), float(atol), _snr(tol_snr)
+
+    def solve(T: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+        n = b.shape[0]
+        if T.shape != (n, n):
+            raise ValueError(f"T must be {n} x {n}, got {tuple(T.shape)}")
+        if T.dtype != torch.float64 or b.dtype != torch.float64:
+            raise TypeError("minsr_pinv_eig works in float64")
+        T, b = T.contiguous(), b.contiguous()
+        need = int(_lib.lib().qtx_pinv_eig_ws_bytes(n))
+        ws = torch.empty(need, dtype=torch.uint8, device=T.device)
+        x = torch.empty(n, dtype=torch.float64, device=T.device)
+        _lib.call("qtx_pinv_eig", _lib.stream(), T.data_ptr(), n, b.data_ptr(), rt, at, ts, x.data_ptr(),
+                  ws.data_ptr(), ws.numel())
+        return x
+
+    return solve
 
 
 def solve_step(O: torch.Tensor, Omean: torch.Tensor, ebar: torch.Tensor, solver: Solver,
                r: Optional[torch.Tensor] = None, ws: Optional[torch.Tensor] = None) -> torch.Tensor:
     """step = solver(Obar, Ebar) with Obar = (O - Omean) sqrt(r / Ns) formed on the fly."""
     ns, np_ = O.shape
-    solver.check(ns, np_)
     od = _lib.QTX_F32 if O.dtype == torch.float32 else _lib.QTX_F64
-    need = int(_lib.lib().qtx_minsr_ws_bytes(ns, np_, solver.mode, _GRAM[solver.gram]))
+    need = solver.ws_bytes(ns, np_)
     if ws is None or ws.numel() < need:
         ws = torch.empty(need, dtype=torch.uint8, device=O.device)
     step = torch.empty(np_, dtype=torch.float64, device=O.device)
-    _lib.call("qtx_minsr_solve", _lib.stream(), od, O.data_ptr(), Omean.data_ptr(), _lib.ptr(r), ebar.data_ptr(),
-              ns, np_, solver.mode, _GRAM[solver.gram], solver.rshift, solver.ashift, solver.rtol, solver.atol,
-              step.data_ptr(), ws.data_ptr(), ws.numel())
+    if solver.snr:
+        _lib.call("qtx_minsr_solve_snr", _lib.stream(), od, O.data_ptr(), Omean.data_ptr(), _lib.ptr(r),
+                  ebar.data_ptr(), ns, np_, solver.cmode, _GRAM[solver.gram], solver.rtol, solver.atol,
+                  solver.tol_snr, step.data_ptr(), ws.data_ptr(), ws.numel())
+    else:
+        _lib.call("qtx_minsr_solve", _lib.stream(), od, O.data_ptr(), Omean.data_ptr(), _lib.ptr(r),
+                  ebar.data_ptr(), ns, np_, solver.cmode, _GRAM[solver.gram], solver.rshift, solver.ashift,
+                  solver.rtol, solver.atol, step.data_ptr(), ws.data_ptr(), ws.numel())
     return step
 
 
@@ -104,7 +158,8 @@ def last_solve_info() -> dict:
     path, steps, lmax, fb = C.c_int(), C.c_int(), C.c_double(), C.c_int()
     _lib.call("qtx_minsr_last_path", C.byref(path), C.byref(steps), C.byref(lmax), C.byref(fb))
     names = {0: "chol_shift", 1: "partial_fractions", 2: "syevd"}
-    reasons = {0: None, 1: "small_cutoff", 2: "nonfinite", 3: "nonpositive_lambda_max", 4: "cholesky_failed"}
+    reasons = {0: None, 1: "small_cutoff", 2: "nonfinite", 3: "nonpositive_lambda_max", 4: "cholesky_failed",
+               5: "snr_truncation"}
     return {"path": names.get(path.value, "none"), "lanczos_steps": steps.value, "lambda_max": lmax.value,
             "fallback": reasons.get(fb.value)}
 
@@ -144,7 +199,7 @@ class QNGD:
         O, mean = self.get_Obar(samples)
         r = None if samples.reweight_factor is None or self._trivial_r(samples) else samples.reweight_factor
         ns, np_ = O.shape
-        need = int(_lib.lib().qtx_minsr_ws_bytes(ns, np_, self._solver.mode, _GRAM[self._solver.gram]))
+        need = self._solver.ws_bytes(ns, np_)
         if self._ws is None or self._ws.numel() < need:
             self._ws = torch.empty(need, dtype=torch.uint8, device=O.device)
         return solve_step(O, mean, ebar, self._solver, r, self._ws)
@@ -207,7 +262,7 @@ class _MomentumSR(SR):
 
     def _sr(self, O, mean, ebar, r):
         ns, np_ = O.shape
-        need = int(_lib.lib().qtx_minsr_ws_bytes(ns, np_, self._solver.mode, _GRAM[self._solver.gram]))
+        need = self._solver.ws_bytes(ns, np_)
         if self._ws is None or self._ws.numel() < need:
             self._ws = torch.empty(need, dtype=torch.uint8, device=O.device)
         return solve_step(O, mean, ebar, self._solver, r, self._ws)

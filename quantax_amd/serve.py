@@ -141,7 +141,7 @@ class JobRun:
         self.applied = torch.empty(1, dtype=torch.int32, device=dev)
         self.energy = torch.full((spec.nsteps,), float("nan"), dtype=torch.float64, device=dev)
         self.solver = auto_pinv_eig(rtol=spec.rtol, gram=gram)
-        nws = int(_lib.lib().qtx_minsr_ws_bytes(ns, spec.nparams, self.solver.mode, _GRAM[gram]))
+        nws = self.solver.ws_bytes(ns, spec.nparams)
         self.ws = torch.empty(max(nws, 1), dtype=torch.uint8, device=dev)
         self.solve_paths: List[str] = []  # soft-inverse route taken at each step
         self.sweep_counter = 0       # Philox step counter of the sweep stream
@@ -213,10 +213,10 @@ class JobRun:
         sp, s = self.spec, self.solver
         t = _lib.QtxMinsrTask()
         t.stream = self.stream.cuda_stream
-        t.o_dtype, t.mode, t.gram = _lib.QTX_F32, s.mode, _GRAM[s.gram]
+        t.o_dtype, t.mode, t.gram = _lib.QTX_F32, s.cmode, _GRAM[s.gram]
         t.O, t.mean, t.r, t.ebar = self.O.data_ptr(), self.mean.data_ptr(), None, self.ebar.data_ptr()
         t.ns, t.np = sp.ns, sp.nparams
-        t.rshift, t.ashift, t.rtol, t.atol = s.rshift, s.ashift, s.rtol, s.atol
+        t.rshift, t.ashift, t.rtol, t.atol, t.tol_snr = s.rshift, s.ashift, s.rtol, s.atol, s.tol_snr
         t.step, t.ws, t.ws_bytes = self.step.data_ptr(), self.ws.data_ptr(), self.ws.numel()
         return t
 
@@ -236,7 +236,7 @@ class JobRun:
         sp, s = self.spec, self.solver
         self.sr_pre()
         _lib.call("qtx_minsr_solve", _lib.stream(), _lib.QTX_F32, self.O.data_ptr(), self.mean.data_ptr(), None,
-                  self.ebar.data_ptr(), sp.ns, sp.nparams, s.mode, _GRAM[s.gram], s.rshift, s.ashift, s.rtol, s.atol,
+                  self.ebar.data_ptr(), sp.ns, sp.nparams, s.cmode, _GRAM[s.gram], s.rshift, s.ashift, s.rtol, s.atol,
                   self.step.data_ptr(), self.ws.data_ptr(), self.ws.numel())
         info = last_solve_info()
         self.sr_post(info["path"] if info["fallback"] is None else f"{info['path']}:{info['fallback']}")

@@ -370,6 +370,67 @@ def test_pinv_solvers_give_minimum_norm_least_squares(rng):
         assert np.allclose(x, ref, rtol=1e-6, atol=1e-9)
 
 
+def test_sum_without_noise_closed_forms():
+    """SNR truncation (solver.py:114-125) on columns whose statistics are
+    known: constant columns have infinite SNR (factor 1), +-a columns sum to
+    zero, and mu +- sigma columns of length m have snr = |mu| sqrt(m) / sigma,
+    so tol = snr halves the sum and twice that divides it by 65; at or below
+    tol 1e-6 nothing changes."""
+    m = 16
+    pm = np.tile([1.0, -1.0], m // 2)
+    c = np.stack([np.full(m, 0.3), 2.5 * pm, 1.0 + 2.0 * pm, -3.0 + 0.5 * pm], axis=1)
+    assert np.array_equal(osr.sum_without_noise(c, 0.0), c.sum(0))
+    assert np.array_equal(osr.sum_without_noise(c, 1e-6), c.sum(0))
+    tol = 1.0 * np.sqrt(m) / 2.0  # the snr of column 2: mu 1, sigma 2
+    x = osr.sum_without_noise(c, tol)
+    assert abs(x[0] - 0.3 * m) < 1e-12 and x[1] == 0.0
+    assert abs(x[2] - m / 2.0) < 1e-12
+    q3 = tol / (3.0 * np.sqrt(m) / 0.5)
+    assert abs(x[3] - (-3.0 * m) / (1 + q3 ** 6)) < 1e-12
+    x2 = osr.sum_without_noise(c, 2 * tol)
+    assert abs(x2[2] - m / 65.0) < 1e-12
+    # invariant under the sign and the scale of a column
+    assert np.allclose(osr.sum_without_noise(-7.0 * c, tol), -7.0 * x, rtol=1e-13, atol=0)
+
+
+def test_snr_truncation_keeps_the_push_through_identity(rng):
+    """With A = P Sigma Q^T, the SR branch's c_sk = (A V)_sk b_s is sigma_k
+    times the MinSR branch's c_ts = U_ts b_t for the same singular pair, and
+    the SNR is scale-free: both branches truncate the same components by the
+    same factors, so minnorm_pinv_eig == lstsq_pinv_eig for any tol_snr.  A
+    sum over the wrong index, or a transposed U, breaks the equality."""
+    for ns, np_ in ((40, 90), (90, 30)):
+        A, b = _problem(rng, ns, np_)
+        for tol in (0.0, 0.3, 1.0, 3.0):
+            x1 = osr.minnorm_pinv_eig(A, b, rtol=1e-10, tol_snr=tol)
+            x2 = osr.lstsq_pinv_eig(A, b, rtol=1e-10, tol_snr=tol)
+            assert np.linalg.norm(x1 - x2) <= 1e-8 * np.linalg.norm(x1), (ns, np_, tol)
+        # a tolerance above every component's SNR leaves almost nothing
+        x0 = osr.auto_pinv_eig(A, b, rtol=1e-10)
+        xs = osr.auto_pinv_eig(A, b, rtol=1e-10, tol_snr=1e4)
+        assert np.linalg.norm(xs) < 1e-6 * np.linalg.norm(x0)
+
+
+def test_minsr_pinv_eig_solves_indefinite_and_matches_minnorm(rng):
+    """minsr_pinv_eig (solver.py:262-294): for a well-conditioned indefinite
+    T and a negligible cutoff it is T^-1 b (the soft inverse keeps the sign of
+    each eigenvalue); on T = A A^T, A^T x is minnorm_pinv_eig(A, b) with the
+    same SNR tolerance."""
+    n = 60
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    d = np.concatenate([rng.uniform(0.5, 2.0, n // 2), -rng.uniform(0.5, 2.0, n - n // 2)])
+    T = (Q * d) @ Q.T
+    T = 0.5 * (T + T.T)
+    b = rng.standard_normal(n)
+    x = osr.minsr_pinv_eig(T, b, rtol=1e-14)
+    assert np.linalg.norm(x - np.linalg.solve(T, b)) < 1e-11 * np.linalg.norm(x)
+    A, bb = _problem(rng, 40, 90)
+    for tol in (0.0, 0.7):
+        y = osr.minsr_pinv_eig(A @ A.T, bb, rtol=1e-10, tol_snr=tol)
+        ref = osr.minnorm_pinv_eig(A, bb, rtol=1e-10, tol_snr=tol)
+        assert np.linalg.norm(A.T @ y - ref) < 1e-9 * np.linalg.norm(ref)
+
+
 def test_eigs_inv_soft_cutoff():
     v = np.array([1e-20, 1e-6, 1.0, 0.0, -2.0])
     inv = osr.eigs_inv(v, rtol=1e-6)
