@@ -842,9 +842,14 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
   stage_weights(1);
   // ---- layer 0 on VALU
   {
-    const float* W0 = pf + off_l0w;
-    const float* b0 = pf + off_l0b;
-    const int8_t* s = sp + b * 256;
+    // this lane's 4 output channels 4 gq + q: weights from one base
+    const float* W0 = pf + off_l0w + 4 * gq * 9;
+    const float* b0 = pf + off_l0b + 4 * gq;
+    // an opaque zero keeps the 9 x RPW spin addresses from being hoisted out
+    // of the forward (they would be held, or spilled, for the whole kernel)
+    int z0 = 0;
+    asm volatile("" : "+s"(z0));
+    const int8_t* s = sp + b * 256 + z0;
 #pragma unroll
     for (int i = 0; i < RPW; ++i) {
       const int y = y0 + i;
@@ -857,10 +862,9 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
       v4f z;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        int co = 4 * gq + q;
-        float a = b0[co];
+        float a = b0[q];
 #pragma unroll
-        for (int tap = 0; tap < 9; ++tap) a += W0[co * 9 + tap] * sv[tap];
+        for (int tap = 0; tap < 9; ++tap) a += W0[q * 9 + tap] * sv[tap];
         z[q] = gelu_f(a, gelu_exact);
       }
       resr[i] = v4f{spin, spin, spin, spin};
@@ -1032,10 +1036,15 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
 // (bf16 hi/lo, 128 bytes per site, 16-byte chunks XORed with site & 7); a
 // layer computes into registers and its outputs replace its inputs between
 // two barriers, which halves the LDS of a ping-pong pair so two workgroups
-// fit on a CU.  The B fragment of (tap, tile) is found through a table of
-// byte offsets of the tap's source site (swizzle key in bits 4-6), built once
-// per launch.  The residual stream stays in registers in the MFMA output
-// layout.  CTW = 1 (six waves, wave pairs sharing tiles, three waves per
+// fit on a CU.  The epilogue runs after the first barrier, one tile at a
+// time (GELU, split, store), so the GELU outputs never coexist with the
+// accumulators; the next layer's first A fragments are fetched after it.
+// That keeps every C = 32 kernel within 256 VGPRs without scratch (GELU
+// before the barrier, with all outputs held, spilled 200-370 bytes per lane
+// and ran the 20 x 20 sweep at 300 against 378 TFLOP/s).  The B fragment of
+// (tap, tile) is found through a table of byte offsets of the tap's source
+// site (swizzle key in bits 4-6), built once per launch.  The residual stream
+// stays in registers in the MFMA output layout.  CTW = 1 (six waves, wave pairs sharing tiles, three waves per
 // SIMD) ran the 20 x 20 sweep at 186-221 TFLOP/s against 301 for CTW = 2:
 // it reads every B fragment twice and issues three MFMAs per LDS round trip.
 constexpr int RC32_MAXN = 400;
@@ -1095,8 +1104,9 @@ __device__ __attribute__((always_inline)) inline void rc_forward_t32(
 
   // ---- layer 0: conv (Cin = 1) of s / sqrt(2) on VALU, GELU, into act
   {
-    const float* W0 = pf + off_l0w;
-    const float* b0 = pf + off_l0b;
+    // this lane's 4 CTW output channels: weights co * 9 + tap from one base
+    const float* W0 = pf + off_l0w + (16 * ct0 + 4 * gq) * 9;
+    const float* b0 = pf + off_l0b + 16 * ct0 + 4 * gq;
 #pragma unroll
     for (int k = 0; k < MAXT; ++k) {
       if (k < nti) {
@@ -1111,7 +1121,7 @@ __device__ __attribute__((always_inline)) inline void rc_forward_t32(
           v4f z;
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
-            const int co = 16 * (ct0 + j) + 4 * gq + q;
+            const int co = 16 * j + q;
             float a = b0[co];
 #pragma unroll
             for (int tap = 0; tap < 9; ++tap) a += W0[co * 9 + tap] * sv[tap];
@@ -1151,10 +1161,7 @@ __device__ __attribute__((always_inline)) inline void rc_forward_t32(
         a[j][0] = an[j][0];
         a[j][1] = an[j][1];
       }
-      if (ks < 8)
-        load_a(l, ks + 1, an);
-      else if (!last)
-        load_a(l + 1, 0, an);
+      if (ks < 8) load_a(l, ks + 1, an);
       const uint16_t* tk = tab + ks * RC32_MAXN + tz;
       // tiles that exist for every wave run unconditionally (sites past N
       // read the table's padding, offset 0, and are never stored); only the
@@ -1177,32 +1184,28 @@ __device__ __attribute__((always_inline)) inline void rc_forward_t32(
     if (!last) {
       const bool is_conv2 = (l % 2) == 1;
       const float scale = is_conv2 ? rsqrtf((float)(l / 2 + 2)) : 1.0f;
-#pragma unroll
-      for (int k = 0; k < MAXT; ++k) {
-#pragma unroll
-        for (int j = 0; j < CTW; ++j) {
-          v4f o;
-          if (is_conv2) {
-            const v4f xr = acc[k][j] + res[k][j];
-            res[k][j] = xr;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) o[q] = gelu_t<GE>(xr[q] * scale);
-          } else {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) o[q] = gelu_t<GE>(acc[k][j][q]);
-          }
-          acc[k][j] = o;
-        }
-        // one tile's GELUs at a time: interleaving all of them costs the
-        // registers of a wave per SIMD
-        __builtin_amdgcn_sched_barrier(0);
-      }
       __syncthreads();  // every wave has read this layer's input
 #pragma unroll
-      for (int k = 0; k < MAXT; ++k)
-        if (k < nti)
+      for (int k = 0; k < MAXT; ++k) {
+        if (k < nti) {
 #pragma unroll
-          for (int j = 0; j < CTW; ++j) store4(k, j, acc[k][j]);
+          for (int j = 0; j < CTW; ++j) {
+            v4f o;
+            if (is_conv2) {
+              const v4f xr = acc[k][j] + res[k][j];
+              res[k][j] = xr;
+#pragma unroll
+              for (int q = 0; q < 4; ++q) o[q] = gelu_t<GE>(xr[q] * scale);
+            } else {
+#pragma unroll
+              for (int q = 0; q < 4; ++q) o[q] = gelu_t<GE>(acc[k][j][q]);
+            }
+            store4(k, j, o);
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      load_a(l + 1, 0, an);
       __syncthreads();
     } else {
       // head: h = (z2 + r) / sqrt(nb + 1); per-lane running logsumexp, one
@@ -1308,6 +1311,13 @@ __global__ __launch_bounds__(W * 64, (ROWS == 2 ? 3 : (ROWS == 3 ? 2 : (W == 8 ?
   if (threadIdx.x < B && c0 + threadIdx.x < ns) logabs[c0 + threadIdx.x] = lp[threadIdx.x];
 }
 
+// log(u) of the acceptance test (metropolis.py:291-322) in FP64, kept out of
+// line: inlined, the constants of FP64 log1p would be hoisted into VGPRs for
+// the whole kernel
+__device__ __noinline__ double accept_threshold(uint32_t w2, uint32_t w3) {
+  return log1p(-words_to_uniform(w2, w3));
+}
+
 template <int C, int MODE, int B, int NMAX, int MAXT, int W, int ROWS>
 __global__ __launch_bounds__(W * 64, (ROWS == 2 ? 3 : (ROWS == 3 ? 2 : (W == 8 ? 2 : 1)))) void rc_metropolis_kernel(
     RcArgs a, int kind, const int32_t* __restrict__ nbr, int width, int hopping, const uint32_t* __restrict__ words,
@@ -1357,7 +1367,7 @@ __global__ __launch_bounds__(W * 64, (ROWS == 2 ? 3 : (ROWS == 3 ? 2 : (W == 8 ?
       if (c < ns) {
         uint4 w = reinterpret_cast<const uint4*>(words)[(int64_t)t * ns + c];
         propose_move(kind, cur + b * N, N, nbr, width, hopping, nhop[b], w.x, w.y, lane, fa, fb, upd);
-        if constexpr (!kThrWave) thr = log1p(-words_to_uniform(w.z, w.w));
+        if constexpr (!kThrWave) thr = accept_threshold(w.z, w.w);
       }
       for (int i = lane; i < N; i += WAVE) {
         int8_t v = cur[b * N + i];
@@ -1378,7 +1388,7 @@ __global__ __launch_bounds__(W * 64, (ROWS == 2 ? 3 : (ROWS == 3 ? 2 : (W == 8 ?
         double thr = 0.0;
         if (c < ns) {
           uint4 w = reinterpret_cast<const uint4*>(words)[(int64_t)t * ns + c];
-          thr = log1p(-words_to_uniform(w.z, w.w));
+          thr = accept_threshold(w.z, w.w);
         }
         if (lane == 0) mv_thr[b] = thr;
       }
@@ -1422,10 +1432,12 @@ __global__ __launch_bounds__(W * 64, (ROWS == 2 ? 3 : (ROWS == 3 ? 2 : (W == 8 ?
 // whose 2^f is one FP32 exp2 (about 1 ulp); the FP64 product d log2(e) and
 // the FP32 rounding of f keep the relative error near 1e-7 for any d, where
 // an FP32 exp of d itself would lose |d| * 2^-24 to the rounding of d alone.
-// Non-finite or huge d take the FP64 exp.
+// |t| >= 2100 is exp's overflow (+inf) or underflow (0) outright, and NaN
+// stays NaN: no FP64 exp, whose polynomial constants would otherwise occupy
+// VGPRs for the whole kernel.
 __device__ __forceinline__ double exp_ratio(double d) {
   const double t = d * 1.4426950408889634;
-  if (!(fabs(t) < 2100.0)) return exp(d);
+  if (!(fabs(t) < 2100.0)) return d > 0.0 ? HUGE_VAL : (d < 0.0 ? 0.0 : d);  // exp's overflow / underflow; NaN stays
   const double k = rint(t);
   return ldexp((double)__builtin_amdgcn_exp2f((float)(t - k)), (int)k);
 }
@@ -1445,7 +1457,8 @@ __global__ __launch_bounds__(W * 64, (ROWS == 2 ? 3 : (ROWS == 3 ? 2 : (W == 8 ?
   __shared__ double lp[B];
   __shared__ int16_t vlist[ROWS == 2 ? RC_MAXCAND_WIN : RC_MAXCAND];
   __shared__ int nvalid_s;
-  __shared__ double diag_s;
+  __shared__ double diag_s, e_s;
+  __shared__ int16_t mv[4 * B];  // slot b: flips applied (2), flips of the next candidate (2)
   const int N = a.g.N;
   const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
   const int64_t n = blockIdx.x;
@@ -1474,51 +1487,66 @@ __global__ __launch_bounds__(W * 64, (ROWS == 2 ? 3 : (ROWS == 3 ? 2 : (W == 8 ?
   const double lp0 = logabs[n];
   // prop[b] holds s0 with candidate start + b applied: thread b < B undoes
   // the previous candidate's (at most two) flips and applies the next one's,
-  // whose sites were fetched before the forward
+  // whose sites were fetched before the forward.  These flips and the sum
+  // live in LDS, not in registers held across the forward.
   for (int i = threadIdx.x; i < B * N; i += (W * WAVE)) prop[i] = s0[i % N];
-  int fa = -1, fb = -1, na = -1, nb2 = -1;
-  if (threadIdx.x < B && (int)threadIdx.x < nvalid) {
-    const int k = vlist[threadIdx.x];
-    na = H.off_a[k];
-    nb2 = H.off_b[k];
+  if (threadIdx.x < B) {
+    const int b = threadIdx.x;
+    int na = -1, nb = -1;
+    if (b < nvalid) {
+      const int k = vlist[b];
+      na = H.off_a[k];
+      nb = H.off_b[k];
+    }
+    mv[4 * b] = (int16_t)na;
+    mv[4 * b + 1] = (int16_t)nb;
   }
+  if (threadIdx.x == 0) e_s = 0.0;
   __syncthreads();
   if (threadIdx.x < B) {
-    int8_t* pb = prop + threadIdx.x * N;
+    const int b = threadIdx.x;
+    int8_t* pb = prop + b * N;
+    const int na = mv[4 * b], nb = mv[4 * b + 1];
     if (na >= 0) pb[na] = -s0[na];
-    if (nb2 >= 0) pb[nb2] = -s0[nb2];
-    fa = na;
-    fb = nb2;
+    if (nb >= 0) pb[nb] = -s0[nb];
   }
   __syncthreads();
-  double e = 0.0;  // accumulated by thread 0 in candidate order
   for (int start = 0; start < nvalid; start += B) {
-    na = nb2 = -1;
-    if (threadIdx.x < B && start + B + (int)threadIdx.x < nvalid) {
-      const int k = vlist[start + B + threadIdx.x];
-      na = H.off_a[k];
-      nb2 = H.off_b[k];
+    if (threadIdx.x < B) {
+      const int b = threadIdx.x;
+      int na = -1, nb = -1;
+      if (start + B + b < nvalid) {
+        const int k = vlist[start + B + b];
+        na = H.off_a[k];
+        nb = H.off_b[k];
+      }
+      mv[4 * b + 2] = (int16_t)na;
+      mv[4 * b + 3] = (int16_t)nb;
     }
     RC_FWD(prop, lp);
     if (threadIdx.x == 0) {
+      double e = e_s;
       for (int b = 0; b < B && start + b < nvalid; ++b) {
         const double r = exp_ratio(lp[b] - lp0);
         e += H.off_c[vlist[start + b]] * r;
       }
+      e_s = e;
     }
     if (threadIdx.x < B) {
-      int8_t* pb = prop + threadIdx.x * N;
+      const int b = threadIdx.x;
+      int8_t* pb = prop + b * N;
+      const int fa = mv[4 * b], fb = mv[4 * b + 1], na = mv[4 * b + 2], nb = mv[4 * b + 3];
       if (fa >= 0) pb[fa] = s0[fa];
       if (fb >= 0) pb[fb] = s0[fb];
       if (na >= 0) pb[na] = -s0[na];
-      if (nb2 >= 0) pb[nb2] = -s0[nb2];
-      fa = na;
-      fb = nb2;
+      if (nb >= 0) pb[nb] = -s0[nb];
+      mv[4 * b] = (int16_t)na;
+      mv[4 * b + 1] = (int16_t)nb;
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    eloc[n] = diag_s + e;
+    eloc[n] = diag_s + e_s;
     if (nconn) nconn[n] = nvalid;
   }
 }
