@@ -63,6 +63,8 @@ def parse():
     p.add_argument("--serve-seq-steps", type=int, default=0,
                    help="steps per job timed for the sequential baselines (0: every step of every job; "
                         "otherwise the rest is extrapolated at the job's later-step cost)")
+    p.add_argument("--activation", default="gelu", choices=["gelu", "relu"],
+                   help="ResConv activation: gelu (the reference's tanh-form GELU) or the opt-in ReLU variant")
     p.add_argument("--calib-copy-mb", type=int, default=0,
                    help="run a device-to-device copy of this many MiB first (HBM counter calibration)")
     return p.parse_args()
@@ -144,7 +146,7 @@ def build_problem(cfg, mode, args, qtx):
     gram = "bf16x3" if mode == "mixed" else "fp64"
     if cfg["model"] == "resconv":
         model = qtx.model.ResConv(cfg["nblocks"], cfg["channels"], precision="bf16x3" if mode == "mixed" else "fp32",
-                                  seed=0)
+                                  seed=0, gelu="relu" if args.activation == "relu" else "tanh")
     else:
         model = qtx.model.RBM_Dense(cfg["features"], dtype=np.float32, init_std=0.01, seed=0)
         gram = "bf16x3" if mode == "mixed" else "fp64"

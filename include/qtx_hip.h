@@ -190,14 +190,20 @@ int qtx_rbm_group_jacobian(void* stream, const qtx_rbm_job* jobs, int32_t njobs,
 /* ---------------------------------------------------------------- ResConv
  * ResConv(nblocks, channels, kernel 3) on an L0 x L1 periodic grid
  * (model/conv_nets.py:26-183, nn/conv.py:13-68).  channels 16 (L0 * L1 <= 512)
- * or 32 (L0 * L1 <= 400), nblocks <= 64.  gelu_exact = 0 uses the tanh form. */
+ * or 32 (L0 * L1 <= 400), nblocks <= 64.  activation: QTX_ACT_GELU_TANH is
+ * the reference's jax.nn.gelu default (model/conv_nets.py:85, 87);
+ * QTX_ACT_GELU_ERF the exact GELU; QTX_ACT_RELU an opt-in ReLU variant (no
+ * counterpart in the reference; derivative 0 at x <= 0, as jax.nn.relu). */
+#define QTX_ACT_GELU_TANH 0
+#define QTX_ACT_GELU_ERF 1
+#define QTX_ACT_RELU 2
 typedef struct qtx_resconv_desc {
   int32_t nblocks;
   int32_t channels;
   int32_t L0;
   int32_t L1;
-  int32_t precision; /* QTX_CONV_* */
-  int32_t gelu_exact;
+  int32_t precision;  /* QTX_CONV_* */
+  int32_t activation; /* QTX_ACT_* */
 } qtx_resconv_desc;
 
 int64_t qtx_resconv_nparams(const qtx_resconv_desc* d);

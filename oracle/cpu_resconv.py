@@ -28,7 +28,10 @@ def resconv_forward_f32(spec, theta, s, approximate=True):
     x = torch.from_numpy(np.asarray(s, np.float32)).reshape(-1, 1, L0, L1)
     params = [(torch.from_numpy(np.asarray(W, np.float32)), torch.from_numpy(np.asarray(B, np.float32)))
               for W, B in spec.unpack(theta)]
-    act = (lambda v: F.gelu(v, approximate="tanh")) if approximate else F.gelu
+    if approximate == "relu":
+        act = F.relu
+    else:
+        act = (lambda v: F.gelu(v, approximate="tanh")) if approximate else F.gelu
 
     def conv(v, W, B):
         return F.conv2d(F.pad(v, (1, 1, 1, 1), mode="circular"), W, B)

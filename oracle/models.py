@@ -65,6 +65,12 @@ def rbm_unpack(theta, M, N):
 
 # ----------------------------------------------------------------- ResConv
 def gelu(x, approximate=True):
+    """GELU, tanh form (approximate=True, jax.nn.gelu's default) or erf form;
+    approximate="relu" selects the opt-in ReLU variant, max(x, 0), which the
+    reference (model/conv_nets.py:85, 87) does not have."""
+    if isinstance(approximate, str):
+        assert approximate == "relu"
+        return np.maximum(x, 0.0)
     if approximate:
         return 0.5 * x * (1.0 + np.tanh(SQRT_2_OVER_PI * (x + 0.044715 * x ** 3)))
     from scipy.special import erf
@@ -72,6 +78,9 @@ def gelu(x, approximate=True):
 
 
 def gelu_grad(x, approximate=True):
+    if isinstance(approximate, str):
+        assert approximate == "relu"
+        return (np.asarray(x) > 0.0).astype(np.float64)  # jax.nn.relu: 0 at x = 0
     if approximate:
         u = SQRT_2_OVER_PI * (x + 0.044715 * x ** 3)
         t = np.tanh(u)

@@ -20,6 +20,7 @@ QTX_GRAM_FP64, QTX_GRAM_FP32, QTX_GRAM_BF16X3 = 0, 1, 2
 QTX_SOLVE_CHOL_SHIFT, QTX_SOLVE_EIGH_PINV, QTX_SOLVE_EIGH_PINV_DIRECT = 0, 1, 2
 QTX_SOLVE_MINNORM, QTX_SOLVE_LSTSQ, QTX_SOLVE_SNR = 0x100, 0x200, 0x400
 QTX_CONV_FP32, QTX_CONV_BF16X3 = 0, 1
+QTX_ACT_GELU_TANH, QTX_ACT_GELU_ERF, QTX_ACT_RELU = 0, 1, 2
 TAG_SWEEP, TAG_INIT = 0x51A7, 0x1D17
 
 vp, i32, i64, u32, u64, f64, sz = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_double, C.c_size_t
@@ -51,7 +52,7 @@ class QtxMinsrTask(C.Structure):
 
 class QtxResConvDesc(C.Structure):
     _fields_ = [("nblocks", i32), ("channels", i32), ("L0", i32), ("L1", i32), ("precision", i32),
-                ("gelu_exact", i32)]
+                ("activation", i32)]
 
 
 # name -> (restype, argtypes)

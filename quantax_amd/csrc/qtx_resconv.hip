@@ -37,7 +37,7 @@ constexpr int RC_MAXN = 1024;
 
 // ---------------------------------------------------------------- layout
 struct RcLayout {
-  int nb, C, L0, L1, N, mode, gelu_exact;
+  int nb, C, L0, L1, N, mode, actk;
   int win;                   // 16 x 16, C = 16: sliding-row-window forward and its K-step order
   int64_t nparams;
   int64_t off_l0w, off_l0b;  // first conv (Cin = 1): W (C, 9), b (C) in floats
@@ -90,7 +90,7 @@ inline RcLayout rc_layout(const qtx_resconv_desc* d) {
   L.L1 = d->L1;
   L.N = d->L0 * d->L1;
   L.mode = d->precision;
-  L.gelu_exact = d->gelu_exact;
+  L.actk = d->activation;
   L.win = (L.C == 16 && L.L0 == 16 && L.L1 == 16) ? 1 : 0;
   L.nparams = rc_nparams(L.nb, L.C);
   int64_t off = L.nparams * 4;  // raw theta copy first
@@ -209,18 +209,25 @@ __global__ void rc_pack_misc_kernel(const float* __restrict__ theta, int nb, int
 }
 
 // ---------------------------------------------------------------- math
+// Activations (qtx_resconv_desc.activation): QTX_ACT_GELU_TANH, the
+// reference's jax.nn.gelu default (model/conv_nets.py:85, 87),
+// QTX_ACT_GELU_ERF, and QTX_ACT_RELU (max(x, 0), derivative 0 at x <= 0, as
+// jax.nn.relu), an opt-in variant with no counterpart in the reference.
 // tanh form: 0.5 x (1 + tanh(k (x + 0.044715 x^3))) = x / (1 + 2^(x (A + B x^2)))
 // with A = -2 k log2(e), B = -2 k 0.044715 log2(e), k = sqrt(2 / pi); native exp2 and rcp
-__device__ __forceinline__ float gelu_f(float x, int exact) {
-  if (exact) return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f));
+__device__ __forceinline__ float act_f(float x, int actk) {
+  if (actk == QTX_ACT_RELU) return fmaxf(x, 0.0f);
+  if (actk == QTX_ACT_GELU_ERF) return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f));
   constexpr float A = -2.0f * 0.79788456080286536f * 1.4426950408889634f;
   constexpr float Bc = A * 0.044715f;
   float e = __builtin_amdgcn_exp2f(x * fmaf(x * x, Bc, A));
   return x * __builtin_amdgcn_rcpf(1.0f + e);
 }
 
-__device__ __forceinline__ float gelu_grad_f(float x, int exact) {
-  if (exact) return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * __expf(-0.5f * x * x) * 0.39894228040143268f;
+__device__ __forceinline__ float act_grad_f(float x, int actk) {
+  if (actk == QTX_ACT_RELU) return x > 0.0f ? 1.0f : 0.0f;
+  if (actk == QTX_ACT_GELU_ERF)
+    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * __expf(-0.5f * x * x) * 0.39894228040143268f;
   float u = 0.79788456080286536f * (x + 0.044715f * x * x * x);
   float t = 1.0f - 2.0f / (1.0f + __expf(2.0f * u));
   return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * 0.79788456080286536f * (1.0f + 0.134145f * x * x);
@@ -337,7 +344,7 @@ __device__ __forceinline__ void conv_item(const char* act_cfg, const RcGeom& g, 
 // layer reads one and writes the other, so one barrier per layer suffices).
 // red: >= 2 * B * W floats.  Items of wave w: w + W * k, k < MAXT.
 template <int C, int MODE, int B, int MAXT, int W>
-__device__ __attribute__((always_inline)) inline void rc_forward(const RcGeom& g, int nb, int gelu_exact, const char* __restrict__ packed,
+__device__ __attribute__((always_inline)) inline void rc_forward(const RcGeom& g, int nb, int actk, const char* __restrict__ packed,
                            int64_t off_l0w, int64_t off_l0b, int64_t bias_off, int64_t off_frag, int64_t frag_bytes,
                            const int8_t* sp, char* act, float* red, double* lp_out) {
   using K = RcK<C, MODE>;
@@ -378,7 +385,7 @@ __device__ __attribute__((always_inline)) inline void rc_forward(const RcGeom& g
             float a = b0[co];
 #pragma unroll
             for (int tap = 0; tap < 9; ++tap) a += W0[co * 9 + tap] * sv[tap];
-            z[q] = gelu_f(a, gelu_exact);
+            z[q] = act_f(a, actk);
           }
           res[k][ct] = v4f{spin, spin, spin, spin};
           if (pv) act_store4<C, MODE>(act + b * act_cfg_bytes, p, 16 * ct + 4 * gq, z);
@@ -442,10 +449,10 @@ __device__ __attribute__((always_inline)) inline void rc_forward(const RcGeom& g
               v4f xr = acc[ct] + res[k][ct];
               res[k][ct] = xr;
 #pragma unroll
-              for (int q = 0; q < 4; ++q) o[q] = gelu_f(xr[q] * scale, gelu_exact);
+              for (int q = 0; q < 4; ++q) o[q] = act_f(xr[q] * scale, actk);
             } else {
 #pragma unroll
-              for (int q = 0; q < 4; ++q) o[q] = gelu_f(acc[ct][q], gelu_exact);
+              for (int q = 0; q < 4; ++q) o[q] = act_f(acc[ct][q], actk);
             }
             if (p < g.N) act_store4<C, MODE>(out + b * act_cfg_bytes, p, 16 * ct + 4 * gq, o);
           }
@@ -518,7 +525,7 @@ namespace qtx {
 // columns and every tap address is a wave-uniform row offset plus a per-lane
 // column constant computed once.  Same LDS layout and results as rc_forward.
 template <int MODE, int B, int MAXT, int W>
-__device__ __attribute__((always_inline)) inline void rc_forward_rows(const RcGeom& g, int nb, int gelu_exact, const char* __restrict__ packed,
+__device__ __attribute__((always_inline)) inline void rc_forward_rows(const RcGeom& g, int nb, int actk, const char* __restrict__ packed,
                                 int64_t off_l0w, int64_t off_l0b, int64_t bias_off, int64_t off_frag,
                                 int64_t frag_bytes, const int8_t* sp, char* act, float* red, double* lp_out) {
   constexpr int C = 16;
@@ -591,7 +598,7 @@ __device__ __attribute__((always_inline)) inline void rc_forward_rows(const RcGe
           float a = b0[co];
 #pragma unroll
           for (int tap = 0; tap < 9; ++tap) a += W0[co * 9 + tap] * sv[tap];
-          z[q] = gelu_f(a, gelu_exact);
+          z[q] = act_f(a, actk);
         }
         res[k] = v4f{spin, spin, spin, spin};
         store4(act + b * cfg_bytes + y * ROWB, z);
@@ -668,10 +675,10 @@ __device__ __attribute__((always_inline)) inline void rc_forward_rows(const RcGe
             v4f xr = acc + res[k];
             res[k] = xr;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) o[q] = gelu_f(xr[q] * scale, gelu_exact);
+            for (int q = 0; q < 4; ++q) o[q] = act_f(xr[q] * scale, actk);
           } else {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) o[q] = gelu_f(acc[q], gelu_exact);
+            for (int q = 0; q < 4; ++q) o[q] = act_f(acc[q], actk);
           }
           store4(out + b * cfg_bytes + y * ROWB, o);
           // keep one item's fragments live at a time (bounds VGPR use)
@@ -754,14 +761,14 @@ constexpr int RC_WSTAGE = 12288;  // one layer's fragments: bf16x3 6 x 64 x 32 B
 constexpr int rc_win_lds(int B) { return RC_WSTAGE; }  // one weight stage
 typedef __attribute__((address_space(3))) void rc_lds_void;
 
-template <bool GE>
-__device__ __forceinline__ float gelu_t(float x) {
-  return gelu_f(x, GE ? 1 : 0);
+template <int ACT>
+__device__ __forceinline__ float act_t(float x) {
+  return act_f(x, ACT);
 }
 
 template <int MODE, int B>
 __device__ __attribute__((always_inline)) inline void rc_forward_win(
-    const RcGeom& g, int nb, int gelu_exact, const char* __restrict__ packed, int64_t off_l0w, int64_t off_l0b,
+    const RcGeom& g, int nb, int actk, const char* __restrict__ packed, int64_t off_l0w, int64_t off_l0b,
     int64_t bias_off, int64_t off_frag, int64_t frag_bytes, const int8_t* sp, char* act, char* wst, float* red,
     double* lp_out) {
   constexpr int C = 16, W = 4 * B, L0 = 16, RPW = 4;
@@ -865,7 +872,7 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
         float a = b0[q];
 #pragma unroll
         for (int tap = 0; tap < 9; ++tap) a += W0[q * 9 + tap] * sv[tap];
-        z[q] = gelu_f(a, gelu_exact);
+        z[q] = act_f(a, actk);
       }
       resr[i] = v4f{spin, spin, spin, spin};
       store4(act + b * CFGB + y * ROWB, z);
@@ -953,8 +960,8 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
     char* out = act + (cur ^ 1) * BUFB + b * CFGB;
     const bool is_conv2 = (l % 2) == 1;
     const float scale = is_conv2 ? rsqrtf((float)(l / 2 + 2)) : 1.0f;
-    auto layer = [&](auto ge_c, auto c2_c) {
-      constexpr bool GE = decltype(ge_c)::value;
+    auto layer = [&](auto act_c, auto c2_c) {
+      constexpr int ACT = decltype(act_c)::value;
       constexpr bool C2 = decltype(c2_c)::value;
       run_layer(l, in, [&](int i, v4f acc) {
         v4f o;
@@ -962,20 +969,26 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
           v4f xr = acc + resr[i];
           resr[i] = xr;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) o[q] = gelu_t<GE>(xr[q] * scale);
+          for (int q = 0; q < 4; ++q) o[q] = act_t<ACT>(xr[q] * scale);
         } else {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) o[q] = gelu_t<GE>(acc[q]);
+          for (int q = 0; q < 4; ++q) o[q] = act_t<ACT>(acc[q]);
         }
         store4(out + (y0 + i) * ROWB, o);
       });
     };
-    if (gelu_exact) {
-      if (is_conv2) layer(std::true_type{}, std::true_type{});
-      else layer(std::true_type{}, std::false_type{});
+    using A0 = std::integral_constant<int, QTX_ACT_GELU_TANH>;
+    using A1 = std::integral_constant<int, QTX_ACT_GELU_ERF>;
+    using A2 = std::integral_constant<int, QTX_ACT_RELU>;
+    if (actk == QTX_ACT_GELU_ERF) {
+      if (is_conv2) layer(A1{}, std::true_type{});
+      else layer(A1{}, std::false_type{});
+    } else if (actk == QTX_ACT_RELU) {
+      if (is_conv2) layer(A2{}, std::true_type{});
+      else layer(A2{}, std::false_type{});
     } else {
-      if (is_conv2) layer(std::false_type{}, std::true_type{});
-      else layer(std::false_type{}, std::false_type{});
+      if (is_conv2) layer(A0{}, std::true_type{});
+      else layer(A0{}, std::false_type{});
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
@@ -1051,7 +1064,7 @@ constexpr int RC32_MAXN = 400;
 constexpr int RC32_CTW = 2;  // output tiles per wave
 constexpr int rc32_tab_bytes(int nmax) { return 9 * nmax * 2; }
 
-template <int MAXT, int W, int CTW, bool GE>
+template <int MAXT, int W, int CTW, int ACT>
 __device__ __attribute__((always_inline)) inline void rc_forward_t32(
     const RcGeom& g, int nb, const char* __restrict__ packed, int64_t off_l0w, int64_t off_l0b,
     int64_t bias_off, int64_t off_frag, int64_t frag_bytes, const int8_t* sp, char* act, const uint16_t* tab,
@@ -1125,7 +1138,7 @@ __device__ __attribute__((always_inline)) inline void rc_forward_t32(
             float a = b0[co];
 #pragma unroll
             for (int tap = 0; tap < 9; ++tap) a += W0[co * 9 + tap] * sv[tap];
-            z[q] = gelu_t<GE>(a);
+            z[q] = act_t<ACT>(a);
           }
           res[k][j] = v4f{spin, spin, spin, spin};
           store4(k, j, z);
@@ -1195,10 +1208,10 @@ __device__ __attribute__((always_inline)) inline void rc_forward_t32(
               const v4f xr = acc[k][j] + res[k][j];
               res[k][j] = xr;
 #pragma unroll
-              for (int q = 0; q < 4; ++q) o[q] = gelu_t<GE>(xr[q] * scale);
+              for (int q = 0; q < 4; ++q) o[q] = act_t<ACT>(xr[q] * scale);
             } else {
 #pragma unroll
-              for (int q = 0; q < 4; ++q) o[q] = gelu_t<GE>(acc[k][j][q]);
+              for (int q = 0; q < 4; ++q) o[q] = act_t<ACT>(acc[k][j][q]);
             }
             store4(k, j, o);
           }
@@ -1266,7 +1279,7 @@ __device__ __forceinline__ void rc32_build_tab(const RcGeom& g, uint16_t* tab) {
 // ---------------------------------------------------------------- kernels
 struct RcArgs {
   RcGeom g;
-  int nb, gelu_exact;
+  int nb, actk;
   const char* packed;
   int64_t off_l0w, off_l0b, bias_off, off_frag, frag_bytes;
 };
@@ -1274,20 +1287,26 @@ struct RcArgs {
 #define RC_FWD(SP, LP)                                                                                        \
   do {                                                                                                        \
     if constexpr (ROWS == 3) {                                                                                \
-      if (a.gelu_exact)                                                                                       \
-        rc_forward_t32<MAXT, W, RC32_CTW, true>(a.g, a.nb, a.packed, a.off_l0w, a.off_l0b, a.bias_off, a.off_frag,           \
-                                   a.frag_bytes, SP, act, reinterpret_cast<const uint16_t*>(wst), red, LP);   \
+      const uint16_t* tab_ = reinterpret_cast<const uint16_t*>(wst);                                          \
+      if (a.actk == QTX_ACT_GELU_ERF)                                                                         \
+        rc_forward_t32<MAXT, W, RC32_CTW, QTX_ACT_GELU_ERF>(a.g, a.nb, a.packed, a.off_l0w, a.off_l0b,         \
+                                                            a.bias_off, a.off_frag, a.frag_bytes, SP, act,     \
+                                                            tab_, red, LP);                                    \
+      else if (a.actk == QTX_ACT_RELU)                                                                        \
+        rc_forward_t32<MAXT, W, RC32_CTW, QTX_ACT_RELU>(a.g, a.nb, a.packed, a.off_l0w, a.off_l0b, a.bias_off, \
+                                                        a.off_frag, a.frag_bytes, SP, act, tab_, red, LP);     \
       else                                                                                                    \
-        rc_forward_t32<MAXT, W, RC32_CTW, false>(a.g, a.nb, a.packed, a.off_l0w, a.off_l0b, a.bias_off, a.off_frag,          \
-                                    a.frag_bytes, SP, act, reinterpret_cast<const uint16_t*>(wst), red, LP);  \
+        rc_forward_t32<MAXT, W, RC32_CTW, QTX_ACT_GELU_TANH>(a.g, a.nb, a.packed, a.off_l0w, a.off_l0b,        \
+                                                             a.bias_off, a.off_frag, a.frag_bytes, SP, act,    \
+                                                             tab_, red, LP);                                   \
     } else if constexpr (ROWS == 2)                                                                           \
-      rc_forward_win<MODE, B>(a.g, a.nb, a.gelu_exact, a.packed, a.off_l0w, a.off_l0b, a.bias_off, a.off_frag, \
+      rc_forward_win<MODE, B>(a.g, a.nb, a.actk, a.packed, a.off_l0w, a.off_l0b, a.bias_off, a.off_frag, \
                            a.frag_bytes, SP, act, wst, red, LP);                                              \
     else if constexpr (ROWS == 1)                                                                             \
-      rc_forward_rows<MODE, B, MAXT, W>(a.g, a.nb, a.gelu_exact, a.packed, a.off_l0w, a.off_l0b, a.bias_off,   \
+      rc_forward_rows<MODE, B, MAXT, W>(a.g, a.nb, a.actk, a.packed, a.off_l0w, a.off_l0b, a.bias_off,   \
                                         a.off_frag, a.frag_bytes, SP, act, red, LP);                          \
     else                                                                                                      \
-      rc_forward<C, MODE, B, MAXT, W>(a.g, a.nb, a.gelu_exact, a.packed, a.off_l0w, a.off_l0b, a.bias_off,     \
+      rc_forward<C, MODE, B, MAXT, W>(a.g, a.nb, a.actk, a.packed, a.off_l0w, a.off_l0b, a.bias_off,     \
                                       a.off_frag, a.frag_bytes, SP, act, red, LP);                            \
   } while (0)
 
@@ -1687,7 +1706,7 @@ __device__ __forceinline__ void jac_conv_grad_w_mfma(const float* __restrict__ d
 }
 
 template <int C, int NMAX>
-__global__ __launch_bounds__(JAC_THREADS) void rc_jacobian_kernel(RcGeom g, int nb, int gelu_exact,
+__global__ __launch_bounds__(JAC_THREADS) void rc_jacobian_kernel(RcGeom g, int nb, int actk,
                                                                   const float* __restrict__ theta,
                                                                   const int8_t* __restrict__ s, int64_t s0,
                                                                   int64_t ns_chunk, int64_t np,
@@ -1727,7 +1746,7 @@ __global__ __launch_bounds__(JAC_THREADS) void rc_jacobian_kernel(RcGeom g, int 
       float xv = bX[e];
       rx[e] = xv;
       float xin = xv * sc;
-      float yv = (i == 0) ? xin * isq2 : gelu_f(xin, gelu_exact);
+      float yv = (i == 0) ? xin * isq2 : act_f(xin, actk);
       bA[e] = yv;
       ry[e] = yv;
     }
@@ -1740,7 +1759,7 @@ __global__ __launch_bounds__(JAC_THREADS) void rc_jacobian_kernel(RcGeom g, int 
       auto epi = [&](int co, int p, float v) {
         v += bias[co];
         rz[co * N + p] = v;
-        const float a = gelu_f(v, gelu_exact);
+        const float a = act_f(v, actk);
         bB[co * N + p] = a;
         ra[co * N + p] = a;
       };
@@ -1801,9 +1820,9 @@ __global__ __launch_bounds__(JAC_THREADS) void rc_jacobian_kernel(RcGeom g, int 
     for (int e = threadIdx.x; e < C * N; e += JAC_THREADS) bA[e] = ra[e];
     __syncthreads();
     jac_conv_grad_w_mfma<C>(bX, bA, C, nbr, N, NMAX, Orow, w2, b2);
-    // da1 -> dz1 = da1 * gelu'(z1), into bB
+    // da1 -> dz1 = da1 * act'(z1), into bB
     jac_conv_bwd_in_mfma<C>(bX, theta + w2, nbr, N, NMAX, [&](int ci, int q, float v) {
-      bB[ci * N + q] = v * gelu_grad_f(rz[ci * N + q], gelu_exact);
+      bB[ci * N + q] = v * act_grad_f(rz[ci * N + q], actk);
     });
     __syncthreads();
     // conv1: input y_i
@@ -1815,7 +1834,7 @@ __global__ __launch_bounds__(JAC_THREADS) void rc_jacobian_kernel(RcGeom g, int 
       const float sc = rsqrtf((float)(i + 1));
       jac_conv_bwd_in_mfma<C>(bB, theta + w1, nbr, N, NMAX, [&](int ci, int q, float v) {
         const float xin = rx[ci * N + q] * sc;
-        bA[ci * N + q] = v * gelu_grad_f(xin, gelu_exact) * sc + bX[ci * N + q];
+        bA[ci * N + q] = v * act_grad_f(xin, actk) * sc + bX[ci * N + q];
       });
       __syncthreads();
       float* t = bX;
@@ -1836,6 +1855,9 @@ static int rc_check(const qtx_resconv_desc* d) {
   QTX_REQUIRE(d->nblocks >= 1 && d->nblocks <= 64, "nblocks out of range (1..64)");
   QTX_REQUIRE(d->L0 >= 1 && d->L1 >= 1 && d->L0 * d->L1 <= RC_MAXN, "L0 * L1 out of range (1..1024)");
   QTX_REQUIRE(d->precision == QTX_CONV_FP32 || d->precision == QTX_CONV_BF16X3, "unknown precision");
+  QTX_REQUIRE(d->activation == QTX_ACT_GELU_TANH || d->activation == QTX_ACT_GELU_ERF ||
+                  d->activation == QTX_ACT_RELU,
+              "unknown activation");
   if (d->channels == 16) QTX_REQUIRE(d->L0 * d->L1 <= 512, "channels = 16 needs L0 * L1 <= 512");
   if (d->channels == 32) QTX_REQUIRE(d->L0 * d->L1 <= 400, "channels = 32 needs L0 * L1 <= 400");
   return QTX_OK;
@@ -1845,7 +1867,7 @@ static RcArgs rc_args(const RcLayout& L, const void* packed) {
   RcArgs a;
   a.g = RcGeom{L.L0, L.L1, L.N, (L.N + 15) / 16, std::log((double)L.C * (double)L.N)};
   a.nb = L.nb;
-  a.gelu_exact = L.gelu_exact;
+  a.actk = L.actk;
   a.packed = (const char*)packed;
   a.off_l0w = L.off_l0w;
   a.off_l0b = L.off_l0b;
@@ -2038,13 +2060,13 @@ int qtx_resconv_jacobian(void* stream, const qtx_resconv_desc* d, const void* pa
     int64_t nc = (ns - s0) < chunk ? (ns - s0) : chunk;
     if (L.C == 16 && L.N <= 256)
       hipLaunchKernelGGL((rc_jacobian_kernel<16, 256>), dim3((unsigned)nc), dim3(JAC_THREADS), 0, st, g, L.nb,
-                         L.gelu_exact, theta, s, s0, nc, L.nparams, O, (float*)ws);
+                         L.actk, theta, s, s0, nc, L.nparams, O, (float*)ws);
     else if (L.C == 16)
       hipLaunchKernelGGL((rc_jacobian_kernel<16, 512>), dim3((unsigned)nc), dim3(JAC_THREADS), 0, st, g, L.nb,
-                         L.gelu_exact, theta, s, s0, nc, L.nparams, O, (float*)ws);
+                         L.actk, theta, s, s0, nc, L.nparams, O, (float*)ws);
     else
       hipLaunchKernelGGL((rc_jacobian_kernel<32, 400>), dim3((unsigned)nc), dim3(JAC_THREADS), 0, st, g, L.nb,
-                         L.gelu_exact, theta, s, s0, nc, L.nparams, O, (float*)ws);
+                         L.actk, theta, s, s0, nc, L.nparams, O, (float*)ws);
     QTX_LAUNCH_CHECK();
   }
   return QTX_OK;

@@ -114,7 +114,11 @@ class RBM_Dense(_Model):
 
 
 class ResConv(_Model):
-    """ResConv(nblocks, channels, 3) on the current square/rectangular grid."""
+    """ResConv(nblocks, channels, 3) on the current square/rectangular grid.
+
+    ``gelu``: "tanh" (the reference's ``jax.nn.gelu`` default,
+    ``model/conv_nets.py:85, 87``), "erf", or "relu" -- an opt-in ReLU in
+    place of GELU, which the reference does not have."""
 
     def __init__(self, nblocks: int, channels: int, kernel_size: int = 3, dtype=np.float32,
                  precision: str = "fp32", gelu: str = "tanh", seed: Optional[int] = None):
@@ -131,7 +135,11 @@ class ResConv(_Model):
         self.desc = _lib.QtxResConvDesc()
         self.desc.nblocks, self.desc.channels = self.nblocks, self.channels
         self.desc.L0, self.desc.L1 = self.L0, self.L1
-        self.desc.gelu_exact = {"tanh": 0, "erf": 1}[gelu]
+        acts = {"tanh": _lib.QTX_ACT_GELU_TANH, "erf": _lib.QTX_ACT_GELU_ERF, "relu": _lib.QTX_ACT_RELU}
+        if gelu not in acts:
+            raise ValueError(f"gelu must be one of {sorted(acts)}")
+        self.activation = gelu
+        self.desc.activation = acts[gelu]
         self.set_precision(precision, repack=False)
         self.nparams = int(_lib.lib().qtx_resconv_nparams(self.desc))
         rng = np.random.default_rng(get_subkeys() if seed is None else seed)

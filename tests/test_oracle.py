@@ -239,7 +239,10 @@ def _torch_resconv_logpsi(spec, theta, s, approximate=True):
     def conv(x, W, B):
         return F.conv2d(F.pad(x, (1, 1, 1, 1), mode="circular"), W) + B[None, :, None, None]
 
-    g = (lambda z: F.gelu(z, approximate="tanh")) if approximate else F.gelu
+    if approximate == "relu":
+        g = F.relu
+    else:
+        g = (lambda z: F.gelu(z, approximate="tanh")) if approximate else F.gelu
     for i in range(spec.nblocks):
         (W1, B1), (W2, B2) = tensors[2 * i], tensors[2 * i + 1]
         r = x
@@ -254,7 +257,7 @@ def _torch_resconv_logpsi(spec, theta, s, approximate=True):
     return la, tensors
 
 
-@pytest.mark.parametrize("approximate", [True, False])
+@pytest.mark.parametrize("approximate", [True, False, "relu"])
 def test_resconv_forward_and_jacobian_match_torch_autograd(approximate, rng):
     import torch
 
