@@ -736,27 +736,33 @@ __device__ __attribute__((always_inline)) inline void rc_forward_rows(const RcGe
   }
 }
 
-// Sliding-row-window forward for C = 16 on a 16 x 16 grid.  Workgroup: 4 B
-// waves, B configurations; wave w owns rows 4 (w % 4) .. 4 (w % 4) + 3 of
-// configuration w / 4 and walks them in order keeping the B fragments of
-// input rows y - 1, y, y + 1 in registers: every fragment depends only on its
-// input row (split-bf16 K-steps 2 dy + pr: pr 0 = taps (dy, 0) + (dy, 1),
-// pr 1 = tap (dy, 2) in both K halves, whose split products over the three dy
-// take five instructions per output row, see the packing; FP32 K-steps = taps), so each
-// row is read from LDS once per wave and layer instead of three times.
+// Sliding-row-window forward for C = 16 on a 16 x 16 grid.  Workgroup:
+// WPC B waves, B configurations; wave w owns the RPW = 16 / WPC rows
+// RPW (w % WPC) .. of configuration w / WPC and walks them in order keeping
+// the B fragments of input rows y - 1, y, y + 1 in registers: every fragment
+// depends only on its input row (split-bf16 K-steps 2 dy + pr: pr 0 = taps
+// (dy, 0) + (dy, 1), pr 1 = tap (dy, 2) in both K halves, whose split
+// products over the three dy take five instructions per output row, see the
+// packing; FP32 K-steps = taps), so each row is read from LDS once per wave
+// and layer instead of three times.
 //
-// Issue structure: the GELU form and the conv kind (conv1 / conv2 with
+// Issue structure: the activation and the conv kind (conv1 / conv2 with
 // residual) are compile-time inside a layer (one branch per layer), and the
 // rows are software-pipelined: row i's MFMA chain and row i - 1's epilogue
-// (residual, GELU, bf16 split, LDS store) are one basic block, so VALU and
-// MFMA work of the same wave can overlap.  Every wave reads layer l's weight
-// fragments from one LDS stage (wst, RC_WSTAGE bytes) into registers at the
-// layer start; after a barrier, layer l + 1's fragments travel HBM/L2 -> LDS
-// by DMA into the same stage while layer l runs.  The residual stream x
-// (FP32, 4 values per lane and row) stays in registers.  With B = 1 a
-// workgroup needs 44 KB of LDS and at most 168 VGPRs per wave, so three run
-// on a CU: each SIMD holds one wave of each, and their per-layer barriers are
-// independent, so one wave's MFMA phase can overlap the others' VALU phases.
+// (residual, activation, bf16 split, LDS store) are one basic block, so VALU
+// and MFMA work of the same wave can overlap.  Every wave reads layer l's
+// weight fragments from one LDS stage (wst, RC_WSTAGE bytes) into registers
+// at the layer start, together with its first two input rows and the row
+// after its last (the latter two belong to the configuration's other waves);
+// after a barrier, layer l + 1's fragments travel HBM/L2 -> LDS by DMA into
+// the same stage while layer l runs, and each wave writes its output rows in
+// place of its input rows (one activation buffer per configuration).  The
+// residual stream x (FP32, 4 values per lane and row) stays in registers.
+// Dispatched with WPC = 2, B = 2 (8 rows per wave, 46-52 KB of LDS and at
+// most 256 VGPRs, two workgroups per CU): against WPC = 4, B = 1 (4 rows,
+// ping-pong buffers, three workgroups per CU at 168 VGPRs) the per-layer
+// prologue and pipeline drain are spread over twice the rows, and E_loc at
+// 16 x 16 runs at 290 against 283 TFLOP/s (352 against 334 with ReLU).
 constexpr int RC_WSTAGE = 12288;  // one layer's fragments: bf16x3 6 x 64 x 32 B; FP32 9216 B
 constexpr int rc_win_lds(int B) { return RC_WSTAGE; }  // one weight stage
 typedef __attribute__((address_space(3))) void rc_lds_void;
@@ -766,19 +772,18 @@ __device__ __forceinline__ float act_t(float x) {
   return act_f(x, ACT);
 }
 
-template <int MODE, int B>
+template <int MODE, int B, int WPC>
 __device__ __attribute__((always_inline)) inline void rc_forward_win(
     const RcGeom& g, int nb, int actk, const char* __restrict__ packed, int64_t off_l0w, int64_t off_l0b,
     int64_t bias_off, int64_t off_frag, int64_t frag_bytes, const int8_t* sp, char* act, char* wst, float* red,
     double* lp_out) {
-  constexpr int C = 16, W = 4 * B, L0 = 16, RPW = 4;
+  constexpr int C = 16, W = WPC * B, L0 = 16, RPW = L0 / WPC;
   constexpr int ROWB = 16 * 64;
   constexpr int CFGB = L0 * ROWB;
-  constexpr int BUFB = CFGB * B;
   constexpr int KSW = (MODE == QTX_CONV_FP32) ? 36 : 6;
   const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
   const int gq = lane / 16, x = lane % 16;
-  const int b = wv / 4, y0 = RPW * (wv % 4);
+  const int b = wv / WPC, y0 = RPW * (wv % WPC);
   const float* pf = reinterpret_cast<const float*>(packed);
 
   // the 16-byte chunks of column c (64 bytes per column) are XORed with
@@ -884,6 +889,12 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
   // one layer: weights from the LDS stage, rows software-pipelined; EPI is
   // called with (row, accumulator)
   auto run_layer = [&](int l, const char* in, auto&& epi) {
+    // rows y0 - 1 and y0 + RPW belong to the configuration's other waves:
+    // read before the barrier below, after which every wave overwrites its
+    // own rows in place
+    Frag Fm = load_row(in, y0 - 1);
+    Frag F0 = load_row(in, y0);
+    Frag Fe = load_row(in, y0 + RPW);
     float wf[(MODE == QTX_CONV_FP32) ? KSW : 1];
     v8bf wh[(MODE == QTX_CONV_FP32) ? 1 : KSW];
     v8bf wl[(MODE == QTX_CONV_FP32) ? 1 : KSW];
@@ -905,15 +916,13 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
     if (l + 1 < 2 * nb) stage_weights(l + 1);
     const float* bias = pf + bias_off + (int64_t)l * C;
     const v4f bias4 = v4f{bias[4 * gq], bias[4 * gq + 1], bias[4 * gq + 2], bias[4 * gq + 3]};
-    Frag Fm = load_row(in, y0 - 1);
-    Frag F0 = load_row(in, y0);
     v4f accp = bias4;
 #pragma unroll
     for (int i = 0; i <= RPW; ++i) {
       v4f acc = bias4;
       Frag Fp;
       if (i < RPW) {
-        Fp = load_row(in, y0 + i + 1);
+        Fp = (i + 1 < RPW) ? load_row(in, y0 + i + 1) : Fe;
         if constexpr (MODE == QTX_CONV_FP32) {
 #pragma unroll
           for (int dy = 0; dy < 3; ++dy) {
@@ -954,10 +963,10 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
     }
   };
 
-  int cur = 0;
+  char* const buf = act + b * CFGB;  // one activation buffer per configuration, updated in place
   for (int l = 1; l < 2 * nb - 1; ++l) {
-    const char* in = act + cur * BUFB + b * CFGB;
-    char* out = act + (cur ^ 1) * BUFB + b * CFGB;
+    const char* in = buf;
+    char* out = buf;
     const bool is_conv2 = (l % 2) == 1;
     const float scale = is_conv2 ? rsqrtf((float)(l / 2 + 2)) : 1.0f;
     auto layer = [&](auto act_c, auto c2_c) {
@@ -992,12 +1001,11 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
-    cur ^= 1;
   }
   // ---- last layer (conv2 of the last block, no bias, no GELU) and the head
   {
     const int l = 2 * nb - 1;
-    const char* in = act + cur * BUFB + b * CFGB;
+    const char* in = buf;
     const float scale = rsqrtf((float)(nb + 1));
     // log-sum-exp of the head: each lane keeps a running (max, sum) over its
     // own values of the wave's rows; one wave reduction at the end
@@ -1027,9 +1035,9 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
       constexpr double kLogCN = 8.317766166719343;  // log(16 channels * 256 sites)
       const int bb = threadIdx.x;
       float m = -INFINITY;
-      for (int w = 4 * bb; w < 4 * bb + 4; ++w) m = fmaxf(m, red[w * 2]);
+      for (int w = WPC * bb; w < WPC * bb + WPC; ++w) m = fmaxf(m, red[w * 2]);
       float ssum = 0.0f;
-      for (int w = 4 * bb; w < 4 * bb + 4; ++w) ssum += red[w * 2 + 1] * __expf(red[w * 2] - m);
+      for (int w = WPC * bb; w < WPC * bb + WPC; ++w) ssum += red[w * 2 + 1] * __expf(red[w * 2] - m);
       lp_out[bb] = (double)m + (double)logf(ssum) - kLogCN;
     }
     __syncthreads();
@@ -1300,7 +1308,7 @@ struct RcArgs {
                                                              a.bias_off, a.off_frag, a.frag_bytes, SP, act,    \
                                                              tab_, red, LP);                                   \
     } else if constexpr (ROWS == 2)                                                                           \
-      rc_forward_win<MODE, B>(a.g, a.nb, a.actk, a.packed, a.off_l0w, a.off_l0b, a.bias_off, a.off_frag, \
+      rc_forward_win<MODE, B, W / B>(a.g, a.nb, a.actk, a.packed, a.off_l0w, a.off_l0b, a.bias_off, a.off_frag, \
                            a.frag_bytes, SP, act, wst, red, LP);                                              \
     else if constexpr (ROWS == 1)                                                                             \
       rc_forward_rows<MODE, B, MAXT, W>(a.g, a.nb, a.actk, a.packed, a.off_l0w, a.off_l0b, a.bias_off,   \
@@ -1311,9 +1319,9 @@ struct RcArgs {
   } while (0)
 
 template <int C, int MODE, int B, int NMAX, int MAXT, int W, int ROWS>
-__global__ __launch_bounds__(W * 64, (ROWS == 2 ? 3 : (ROWS == 3 ? 2 : (W == 8 ? 2 : 1)))) void rc_logpsi_kernel(RcArgs a, const int8_t* __restrict__ s, int64_t ns,
+__global__ __launch_bounds__(W * 64, (ROWS == 2 ? (W == 4 * B ? 3 : 2) : (ROWS == 3 ? 2 : (W == 8 ? 2 : 1)))) void rc_logpsi_kernel(RcArgs a, const int8_t* __restrict__ s, int64_t ns,
                                                                double* __restrict__ logabs) {
-  __shared__ __attribute__((aligned(16))) char act[(ROWS == 3 ? 1 : 2) * B * NMAX * C * 4];
+  __shared__ __attribute__((aligned(16))) char act[(ROWS >= 2 ? 1 : 2) * B * NMAX * C * 4];
   __shared__ __attribute__((aligned(16))) char wst[ROWS == 2 ? rc_win_lds(B) : (ROWS == 3 ? rc32_tab_bytes(NMAX) : 16)];
   __shared__ int8_t sp[B * NMAX];
   __shared__ float red[2 * B * W];
@@ -1338,11 +1346,11 @@ __device__ __noinline__ double accept_threshold(uint32_t w2, uint32_t w3) {
 }
 
 template <int C, int MODE, int B, int NMAX, int MAXT, int W, int ROWS>
-__global__ __launch_bounds__(W * 64, (ROWS == 2 ? 3 : (ROWS == 3 ? 2 : (W == 8 ? 2 : 1)))) void rc_metropolis_kernel(
+__global__ __launch_bounds__(W * 64, (ROWS == 2 ? (W == 4 * B ? 3 : 2) : (ROWS == 3 ? 2 : (W == 8 ? 2 : 1)))) void rc_metropolis_kernel(
     RcArgs a, int kind, const int32_t* __restrict__ nbr, int width, int hopping, const uint32_t* __restrict__ words,
     int nsteps, int64_t ns, double reweight, double tie_tol, int8_t* __restrict__ s, double* __restrict__ logabs,
     int32_t* __restrict__ n_accept, int32_t* __restrict__ n_near) {
-  __shared__ __attribute__((aligned(16))) char act[(ROWS == 3 ? 1 : 2) * B * NMAX * C * 4];
+  __shared__ __attribute__((aligned(16))) char act[(ROWS >= 2 ? 1 : 2) * B * NMAX * C * 4];
   __shared__ __attribute__((aligned(16))) char wst[ROWS == 2 ? rc_win_lds(B) : (ROWS == 3 ? rc32_tab_bytes(NMAX) : 16)];
   __shared__ int8_t cur[B * NMAX];
   __shared__ int8_t prop[B * NMAX];
@@ -1465,10 +1473,10 @@ constexpr int RC_MAXCAND = 4096;
 constexpr int RC_MAXCAND_WIN = 3072;  // 16 x 16 window kernels: keeps two workgroups' LDS within a CU
 
 template <int C, int MODE, int B, int NMAX, int MAXT, int W, int ROWS>
-__global__ __launch_bounds__(W * 64, (ROWS == 2 ? 3 : (ROWS == 3 ? 2 : (W == 8 ? 2 : 1)))) void rc_eloc_kernel(RcArgs a, qtx_ham H, const int8_t* __restrict__ s,
+__global__ __launch_bounds__(W * 64, (ROWS == 2 ? (W == 4 * B ? 3 : 2) : (ROWS == 3 ? 2 : (W == 8 ? 2 : 1)))) void rc_eloc_kernel(RcArgs a, qtx_ham H, const int8_t* __restrict__ s,
                                                              const double* __restrict__ logabs, int64_t ns,
                                                              double* __restrict__ eloc, int32_t* __restrict__ nconn) {
-  __shared__ __attribute__((aligned(16))) char act[(ROWS == 3 ? 1 : 2) * B * NMAX * C * 4];
+  __shared__ __attribute__((aligned(16))) char act[(ROWS >= 2 ? 1 : 2) * B * NMAX * C * 4];
   __shared__ __attribute__((aligned(16))) char wst[ROWS == 2 ? rc_win_lds(B) : (ROWS == 3 ? rc32_tab_bytes(NMAX) : 16)];
   __shared__ int8_t s0[NMAX];
   __shared__ int8_t prop[B * NMAX];
@@ -1882,13 +1890,13 @@ static RcArgs rc_args(const RcLayout& L, const void* packed) {
 #define RC_DISPATCH(KERNEL, NS_PER_BLOCK_OUT, GRID_FROM_B, ...)                                        \
   do {                                                                                                 \
     if (L.win) {                                                                                       \
-      constexpr int B_ = 1;                                                                            \
+      constexpr int B_ = 2;                                                                            \
       NS_PER_BLOCK_OUT = B_;                                                                           \
       dim3 grid_ = GRID_FROM_B;                                                                        \
       if (L.mode == QTX_CONV_FP32)                                                                     \
-        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_FP32, 1, 256, 4, 4, 2>), grid_, dim3(256), 0, st, __VA_ARGS__); \
+        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_FP32, 2, 256, 4, 4, 2>), grid_, dim3(256), 0, st, __VA_ARGS__); \
       else                                                                                             \
-        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_BF16X3, 1, 256, 4, 4, 2>), grid_, dim3(256), 0, st, __VA_ARGS__); \
+        hipLaunchKernelGGL((KERNEL<16, QTX_CONV_BF16X3, 2, 256, 4, 4, 2>), grid_, dim3(256), 0, st, __VA_ARGS__); \
     } else if (L.C == 16 && L.L1 == 16 && L.L0 <= 16) {                                                \
       constexpr int B_ = 2;                                                                            \
       NS_PER_BLOCK_OUT = B_;                                                                           \
