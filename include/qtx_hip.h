@@ -53,6 +53,12 @@ extern "C" {
 #define QTX_SOLVE_MINNORM 0x100
 #define QTX_SOLVE_LSTSQ 0x200
 #define QTX_SOLVE_SNR 0x400
+/* flag: the workspace is qtx_minsr_ws_bytes(...) bytes for this library
+ * followed by the rocBLAS / rocSOLVER scratch of the solve (at least
+ * qtx_minsr_blas_ws_bytes(...) bytes): the library's handles then allocate
+ * nothing.  Without it that scratch is allocated by rocBLAS, once per
+ * (thread, device, stream) handle, and kept. */
+#define QTX_SOLVE_BLAS_WS 0x800
 
 /* ResConv forward precision */
 #define QTX_CONV_FP32 0   /* FP32 products on FP32 MFMA */
@@ -322,6 +328,18 @@ typedef struct qtx_minsr_task {
                            needs QTX_SOLVE_SNR in the mode of qtx_minsr_ws_bytes), 0: none */
 } qtx_minsr_task;
 int qtx_minsr_solve_many(qtx_minsr_task* tasks, int32_t ntasks);
+
+/* The rocBLAS / rocSOLVER scratch bytes of a solve of this shape and mode on
+ * this stream, by a rocBLAS device-memory size query over every library call
+ * the solve can make (no kernels run).  ws: a workspace of at least
+ * qtx_minsr_ws_bytes(ns, np, mode, gram) bytes (pointers into it are handed to
+ * the query; nothing is read or written).  *bytes is at least 256 (a solve
+ * with QTX_SOLVE_BLAS_WS needs a non-empty tail).  A workspace of
+ * qtx_minsr_ws_bytes + *bytes bytes with QTX_SOLVE_BLAS_WS in the mode makes
+ * the solve allocate nothing; a tail too small for a rocSOLVER call fails
+ * that solve with QTX_EWORKSPACE.  No counterpart in the reference (XLA owns its scratch). */
+int qtx_minsr_blas_ws_bytes(void* stream, int64_t ns, int64_t np, int mode, int gram, void* ws, size_t ws_bytes,
+                            size_t* bytes);
 
 /* qtx_minsr_solve with the signal-to-noise truncation of minnorm_pinv_eig /
  * lstsq_pinv_eig (optimizer/solver.py:114-164, _sum_without_noise): each

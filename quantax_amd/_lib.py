@@ -18,7 +18,7 @@ QTX_F32, QTX_F64 = 0, 1
 QTX_LOCALFLIP, QTX_EXCHANGE = 0, 1
 QTX_GRAM_FP64, QTX_GRAM_FP32, QTX_GRAM_BF16X3 = 0, 1, 2
 QTX_SOLVE_CHOL_SHIFT, QTX_SOLVE_EIGH_PINV, QTX_SOLVE_EIGH_PINV_DIRECT = 0, 1, 2
-QTX_SOLVE_MINNORM, QTX_SOLVE_LSTSQ, QTX_SOLVE_SNR = 0x100, 0x200, 0x400
+QTX_SOLVE_MINNORM, QTX_SOLVE_LSTSQ, QTX_SOLVE_SNR, QTX_SOLVE_BLAS_WS = 0x100, 0x200, 0x400, 0x800
 QTX_CONV_FP32, QTX_CONV_BF16X3 = 0, 1
 QTX_ACT_GELU_TANH, QTX_ACT_GELU_ERF, QTX_ACT_RELU = 0, 1, 2
 TAG_SWEEP, TAG_INIT = 0x51A7, 0x1D17
@@ -98,6 +98,7 @@ PROTOS = {
     "qtx_minsr_ws_bytes": (sz, [i64, i64, C.c_int, C.c_int]),
     "qtx_minsr_solve": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, i64, i64, C.c_int, C.c_int, f64, f64, f64, f64, vp,
                                   vp, sz]),
+    "qtx_minsr_blas_ws_bytes": (C.c_int, [vp, i64, i64, C.c_int, C.c_int, vp, sz, C.POINTER(C.c_size_t)]),
     "qtx_minsr_solve_many": (C.c_int, [C.POINTER(QtxMinsrTask), i32]),
     "qtx_minsr_solve_snr": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, i64, i64, C.c_int, C.c_int, f64, f64, f64, vp,
                                       vp, sz]),

@@ -794,44 +794,54 @@ __global__ void snr_scale_kernel(double* __restrict__ rho, const double* __restr
 // One handle per (thread, device, stream).  A handle carries the device
 // workspace rocSOLVER uses for its temporaries, so one handle switched between
 // streams would let two factorisations in flight on different streams share
-// that scratch; a handle bound to a single stream cannot.  The cache keeps at
-// most kMaxHandles per thread and device; the oldest is retired after a
-// device-wide sync.
+// that scratch; a handle bound to a single stream cannot.  Two caches: handles
+// whose scratch rocBLAS allocates and keeps (the default), and handles that
+// only ever use scratch the caller brings in its workspace (QTX_SOLVE_BLAS_WS:
+// rocblas_set_workspace before every use, so rocBLAS allocates nothing).  Each
+// keeps at most kMaxHandles per thread and device; the oldest is retired after
+// a device-wide sync.
 constexpr size_t kMaxHandles = 64;
 struct BlasEntry {
   hipStream_t stream;
   rocblas_handle handle;
 };
 static thread_local std::vector<BlasEntry> g_blas[64];
+static thread_local std::vector<BlasEntry> g_blas_user[64];
 
-static int get_blas(hipStream_t st, rocblas_handle* h) {
+static int get_blas(hipStream_t st, rocblas_handle* h, void* user_ws = nullptr, size_t user_bytes = 0,
+                    bool user = false) {
   int dev = 0;
   QTX_HIP(hipGetDevice(&dev));
   if (dev < 0 || dev >= 64) return fail(QTX_EHIP, "device index out of range");
-  std::vector<BlasEntry>& cache = g_blas[dev];
+  std::vector<BlasEntry>& cache = user ? g_blas_user[dev] : g_blas[dev];
+  rocblas_handle found = nullptr;
   for (const BlasEntry& e : cache)
-    if (e.stream == st) {
-      *h = e.handle;
-      return QTX_OK;
+    if (e.stream == st) found = e.handle;
+  if (!found) {
+    if (cache.size() >= kMaxHandles) {
+      QTX_HIP(hipDeviceSynchronize());
+      rocblas_destroy_handle(cache.front().handle);
+      cache.erase(cache.begin());
     }
-  if (cache.size() >= kMaxHandles) {
-    QTX_HIP(hipDeviceSynchronize());
-    rocblas_destroy_handle(cache.front().handle);
-    cache.erase(cache.begin());
+    if (rocblas_create_handle(&found) != rocblas_status_success)
+      return fail(QTX_ESOLVER, "rocblas_create_handle failed");
+    if (rocblas_set_stream(found, st) != rocblas_status_success) {
+      rocblas_destroy_handle(found);
+      return fail(QTX_ESOLVER, "rocblas_set_stream failed");
+    }
+    cache.push_back(BlasEntry{st, found});
   }
-  rocblas_handle nh = nullptr;
-  if (rocblas_create_handle(&nh) != rocblas_status_success) return fail(QTX_ESOLVER, "rocblas_create_handle failed");
-  if (rocblas_set_stream(nh, st) != rocblas_status_success) {
-    rocblas_destroy_handle(nh);
-    return fail(QTX_ESOLVER, "rocblas_set_stream failed");
-  }
-  cache.push_back(BlasEntry{st, nh});
-  *h = nh;
+  if (user && rocblas_set_workspace(found, user_ws, user_bytes) != rocblas_status_success)
+    return fail(QTX_ESOLVER, "rocblas_set_workspace failed");
+  *h = found;
   return QTX_OK;
 }
 
 static int rb(rocblas_status s, const char* what) {
   if (s == rocblas_status_success) return QTX_OK;
+  if (s == rocblas_status_memory_error)
+    return fail(QTX_EWORKSPACE, std::string(what) + ": rocBLAS/rocSOLVER scratch does not fit (with "
+                                "QTX_SOLVE_BLAS_WS, size the workspace tail by qtx_minsr_blas_ws_bytes)");
   return fail(QTX_ESOLVER, std::string(what) + " failed: " + rocblas_status_to_string(s));
 }
 
@@ -1331,6 +1341,8 @@ struct Task {
   qtx_minsr_task* io = nullptr;
   hipStream_t st = nullptr;
   SolveWs w{};
+  char* blas_ws = nullptr;  // QTX_SOLVE_BLAS_WS: rocBLAS scratch = the workspace past the library's own part
+  size_t blas_bytes = 0;
   bool minnorm = false;
   int64_t n = 0;
   bool failed = false;
@@ -1602,7 +1614,7 @@ static int check_task(const qtx_minsr_task& io) {
   const int mode = base_mode(io.mode);
   QTX_REQUIRE(mode == QTX_SOLVE_CHOL_SHIFT || mode == QTX_SOLVE_EIGH_PINV || mode == QTX_SOLVE_EIGH_PINV_DIRECT,
               "unknown solver mode");
-  QTX_REQUIRE((io.mode & ~(0xff | QTX_SOLVE_MINNORM | QTX_SOLVE_LSTSQ | QTX_SOLVE_SNR)) == 0 &&
+  QTX_REQUIRE((io.mode & ~(0xff | QTX_SOLVE_MINNORM | QTX_SOLVE_LSTSQ | QTX_SOLVE_SNR | QTX_SOLVE_BLAS_WS)) == 0 &&
                   !((io.mode & QTX_SOLVE_MINNORM) && (io.mode & QTX_SOLVE_LSTSQ)),
               "bad solver mode flags");
   QTX_REQUIRE(io.gram == QTX_GRAM_FP64 || io.gram == QTX_GRAM_FP32 || io.gram == QTX_GRAM_BF16X3,
@@ -1611,8 +1623,67 @@ static int check_task(const qtx_minsr_task& io) {
   QTX_REQUIRE(!(io.tol_snr > kSnrMin) || mode == QTX_SOLVE_CHOL_SHIFT || (io.mode & QTX_SOLVE_SNR),
               "tol_snr > 0 needs QTX_SOLVE_SNR in the mode (and its workspace)");
   size_t need = ws_layout(io.ns, io.np, io.mode, io.gram, nullptr, nullptr);
+  if (io.mode & QTX_SOLVE_BLAS_WS) need += 256;  // a non-empty rocBLAS scratch tail
   if (io.ws == nullptr || io.ws_bytes < need)
     return fail(QTX_EWORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+  return QTX_OK;
+}
+
+// rocBLAS / rocSOLVER scratch of one solve: a device-memory size query (no
+// kernels run, nothing is read or written) over every library call the solve
+// can make, with its dimensions and with pointers into the caller's workspace
+static int blas_query(rocblas_handle h, int64_t ns, int64_t np, int mode, SolveWs& w, size_t* bytes) {
+  const bool mn = use_minnorm(ns, np, mode);
+  const int64_t n = mn ? ns : np, nn = n * n;
+  const rocblas_int ni = (rocblas_int)n;
+  const int bm = base_mode(mode);
+  const double one = 1.0, zero = 0.0, minus_one = -1.0;
+  if (rocblas_start_device_memory_size_query(h) != rocblas_status_success)
+    return fail(QTX_ESOLVER, "rocblas_start_device_memory_size_query failed");
+  auto ok = [](rocblas_status s) {
+    return s == rocblas_status_success || s == rocblas_status_size_increased || s == rocblas_status_size_unchanged;
+  };
+  bool good = true;
+  if (!mn) {  // SR branch: S by row blocks of Obar, and the SNR products
+    const rocblas_int npi = (rocblas_int)np, rows = (rocblas_int)obar_rows(ns, np);
+    good &= ok(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, npi, npi, rows, &one, w.obar,
+                             npi, w.obar, npi, &zero, w.T, npi));
+    if (mode & QTX_SOLVE_SNR)
+      good &= ok(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, npi, rows, npi, &one, w.T, npi,
+                               w.obar, npi, &zero, w.cbuf, npi));
+  }
+  if (bm == QTX_SOLVE_CHOL_SHIFT || bm == QTX_SOLVE_EIGH_PINV) {  // panels of the blocked Cholesky
+    double* A = (bm == QTX_SOLVE_CHOL_SHIFT) ? w.T : w.T2;
+    for (int64_t k = 0; k < n; k += kCholBlock) {
+      const int64_t kb = (n - k < kCholBlock) ? n - k : kCholBlock, m = n - k - kb;
+      if (m == 0) break;
+      double* Akk = A + k * n + k;
+      good &= ok(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                               rocblas_diagonal_non_unit, (rocblas_int)m, (rocblas_int)kb, &one, Akk, ni, Akk + kb,
+                               ni));
+      good &= ok(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, (rocblas_int)m, (rocblas_int)kb,
+                               &minus_one, Akk + kb, ni, &one, A + (k + kb) * n + (k + kb), ni));
+    }
+  }
+  if (bm == QTX_SOLVE_CHOL_SHIFT) {
+    good &= ok(rocsolver_dpotrs(h, rocblas_fill_lower, ni, 1, w.T, ni, w.y, ni));
+  } else {  // the dsyevd route (EIGH_PINV: its fallback)
+    good &= ok(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, ni, w.T, ni, w.D, w.E, w.info));
+    good &= ok(rocblas_dgemv(h, rocblas_operation_transpose, ni, ni, &one, w.T, ni, w.y, 1, &zero, w.z, 1));
+    good &= ok(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &one, w.T, ni, w.z, 1, &zero, w.y, 1));
+  }
+  if (bm == QTX_SOLVE_EIGH_PINV) {  // partial fractions: T^2 and the refinement's triangular solves
+    good &= ok(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, ni, ni, &one, w.T, ni, &zero, w.T2, ni));
+    for (rocblas_operation op : {rocblas_operation_none, rocblas_operation_transpose})
+      good &= ok(rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_lower, op, rocblas_diagonal_non_unit,
+                                               ni, 2, &one, w.T2, ni, nn, w.R, ni, 2 * n, 3));
+  }
+  size_t sz = 0;
+  const rocblas_status st = rocblas_stop_device_memory_size_query(h, &sz);
+  if (!good || st != rocblas_status_success) return fail(QTX_ESOLVER, "rocBLAS device memory size query failed");
+  // at least 256 bytes: a zero-byte workspace would put the handle back under
+  // rocBLAS's own memory management
+  *bytes = std::max<size_t>(256, (sz + 255) & ~(size_t)255);
   return QTX_OK;
 }
 
@@ -1651,13 +1722,18 @@ static int solve_many(qtx_minsr_task* tasks, int ntasks) {
     t.st = as_stream(tasks[i].stream);
     t.minnorm = use_minnorm(tasks[i].ns, tasks[i].np, tasks[i].mode);
     t.n = t.minnorm ? tasks[i].ns : tasks[i].np;
-    ws_layout(tasks[i].ns, tasks[i].np, tasks[i].mode, tasks[i].gram, &t.w, (char*)tasks[i].ws);
+    const size_t own = ws_layout(tasks[i].ns, tasks[i].np, tasks[i].mode, tasks[i].gram, &t.w, (char*)tasks[i].ws);
+    if (tasks[i].mode & QTX_SOLVE_BLAS_WS) {
+      t.blas_ws = (char*)tasks[i].ws + own;
+      t.blas_bytes = tasks[i].ws_bytes - own;
+    }
   }
   rocblas_handle h = nullptr;
   for (int phase = 1; phase <= 3; ++phase) {
     for (Task& t : ts) {
       if (t.failed) continue;
-      int rc = get_blas(t.st, &h);
+      const bool user = (t.io->mode & QTX_SOLVE_BLAS_WS) != 0;
+      int rc = get_blas(t.st, &h, t.blas_ws, t.blas_bytes, user);
       if (rc == QTX_OK) {
         rocblas_pointer_mode pm;
         rocblas_get_pointer_mode(h, &pm);
@@ -1839,6 +1915,28 @@ int qtx_apply_update(void* stream, int dtype, void* theta, const double* step, d
                        lr, np, applied);
   QTX_LAUNCH_CHECK();
   return QTX_OK;
+}
+
+int qtx_minsr_blas_ws_bytes(void* stream, int64_t ns, int64_t np, int mode, int gram, void* ws, size_t ws_bytes,
+                            size_t* bytes) {
+  QTX_REQUIRE(ns > 0 && np > 0 && bytes, "bad arguments");
+  const int bm = base_mode(mode);
+  QTX_REQUIRE(bm == QTX_SOLVE_CHOL_SHIFT || bm == QTX_SOLVE_EIGH_PINV || bm == QTX_SOLVE_EIGH_PINV_DIRECT,
+              "unknown solver mode");
+  const size_t own = ws_layout(ns, np, mode, gram, nullptr, nullptr);
+  if (ws == nullptr || ws_bytes < own)
+    return fail(QTX_EWORKSPACE, "workspace too small: need " + std::to_string(own) + " bytes");
+  SolveWs w{};
+  ws_layout(ns, np, mode, gram, &w, (char*)ws);
+  rocblas_handle h = nullptr;
+  int rc = get_blas(as_stream(stream), &h, nullptr, 0, true);
+  if (rc) return rc;
+  rocblas_pointer_mode pm;
+  rocblas_get_pointer_mode(h, &pm);
+  rocblas_set_pointer_mode(h, rocblas_pointer_mode_host);
+  rc = blas_query(h, ns, np, mode, w, bytes);
+  rocblas_set_pointer_mode(h, pm);
+  return rc;
 }
 
 int qtx_minsr_solve_snr(void* stream, int o_dtype, const void* O, const double* mean, const double* r,

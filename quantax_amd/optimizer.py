@@ -40,6 +40,9 @@ class Solver:
     gram: str = "fp64"
     branch: str = "auto"  # "auto" | "minnorm" | "lstsq": the reference factory this came from
     tol_snr: float = 0.0  # SNR truncation of the eigen-components (pinv solvers), <= 1e-6: none
+    # rocBLAS / rocSOLVER scratch in the solve's workspace (torch's allocator)
+    # rather than allocated by rocBLAS per stream handle
+    caller_scratch: bool = True
 
     @property
     def snr(self) -> bool:
@@ -48,7 +51,8 @@ class Solver:
     @property
     def cmode(self) -> int:
         """The mode word of the C-ABI: the base mode, the branch flag of a
-        minnorm_* / lstsq_* factory and the SNR workspace flag."""
+        minnorm_* / lstsq_* factory, the SNR workspace flag and the
+        caller-scratch flag."""
         m = self.mode
         if self.branch == "minnorm":
             m |= _lib.QTX_SOLVE_MINNORM
@@ -56,10 +60,32 @@ class Solver:
             m |= _lib.QTX_SOLVE_LSTSQ
         if self.snr:
             m |= _lib.QTX_SOLVE_SNR
+        if self.caller_scratch:
+            m |= _lib.QTX_SOLVE_BLAS_WS
         return m
 
     def ws_bytes(self, ns: int, np_: int) -> int:
-        return int(_lib.lib().qtx_minsr_ws_bytes(ns, np_, self.cmode, _GRAM[self.gram]))
+        """Workspace bytes of a solve of this shape: the library's own part,
+        plus (caller_scratch) the rocBLAS / rocSOLVER scratch by a size query
+        on the current stream's handle, cached per shape and mode."""
+        own = int(_lib.lib().qtx_minsr_ws_bytes(ns, np_, self.cmode, _GRAM[self.gram]))
+        if not self.caller_scratch:
+            return own
+        return own + _blas_bytes(ns, np_, self.cmode, _GRAM[self.gram], own)
+
+
+_BLAS_BYTES = {}
+
+
+def _blas_bytes(ns: int, np_: int, cmode: int, gram: int, own: int) -> int:
+    key = (ns, np_, cmode, gram)
+    if key not in _BLAS_BYTES:
+        tmp = torch.empty(max(own, 1), dtype=torch.uint8, device=_lib.device())
+        out = C.c_size_t(0)
+        _lib.call("qtx_minsr_blas_ws_bytes", _lib.stream(), ns, np_, cmode, gram, tmp.data_ptr(), tmp.numel(),
+                  C.byref(out))
+        _BLAS_BYTES[key] = int(out.value)
+    return _BLAS_BYTES[key]
 
 
 def _rt(v):

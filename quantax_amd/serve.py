@@ -9,8 +9,11 @@ RBM width and sample count.
 * Admission: a job is admitted (FIFO) once it has arrived and its device
   footprint (O, the solver workspace, chains) fits the HBM budget next to the
   jobs already running; finished jobs release their memory at once.
-* Continuous batching: every round advances every admitted job by one VMC
-  step.  Sampling, E_loc and the Jacobian of all jobs are grouped launches
+* Continuous batching: every round advances every admitted job by at least
+  one VMC step; a job whose step costs 1/k of the largest active job's takes
+  min(k, pacing) steps, in sub-rounds that group only the jobs with steps
+  left in the round, so small jobs are not held in lock-step with the
+  largest.  Sampling, E_loc and the Jacobian of all jobs are grouped launches
   (``qtx_rbm_group_*``: one persistent launch per register class), so a job
   that arrives while others run joins the next round's launches.  The per-job
   SR solves (Gram, soft inverse, update) then run on per-job streams, all
@@ -102,7 +105,7 @@ def c5_stream(njobs: int = 64, mean_interarrival: float = 0.25, nsteps: int = 50
 def footprint_bytes(spec: JobSpec, gram: str) -> int:
     """Device bytes a job holds while admitted (O, solver workspace, chains)."""
     ns, npar, N = spec.ns, spec.nparams, spec.N
-    ws = int(_lib.lib().qtx_minsr_ws_bytes(ns, npar, _lib.QTX_SOLVE_EIGH_PINV, _GRAM[gram]))
+    ws = auto_pinv_eig(rtol=spec.rtol, gram=gram).ws_bytes(ns, npar)  # with the rocSOLVER scratch
     o_bytes = ns * npar * 4
     small = ns * (N + 8 * 4 + 4 * 2) + npar * (4 * 2 + 8 * 2) + (1 << 20)
     return o_bytes + ws + small
@@ -282,13 +285,17 @@ class Server:
     """Continuous-batching VMC server on the current GPU."""
 
     def __init__(self, hbm_budget: Optional[int] = None, gram: str = "bf16x3", keep_results: bool = False,
-                 progress=None, max_active: Optional[int] = None):
+                 progress=None, max_active: Optional[int] = None, pacing: int = 8):
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info()
             hbm_budget = int(0.7 * free)
         self.budget = int(hbm_budget)
         self.gram = gram
         self.max_active = max_active  # optional cap on concurrent jobs (lower median latency)
+        # per-job pacing: a job whose step costs 1/k of the largest active
+        # job's runs min(k, pacing) steps per round (sub-rounds of the smaller
+        # jobs only), so small jobs are not held in lock-step with the largest
+        self.pacing = max(1, int(pacing))
         self.keep_results = keep_results
         self.progress = progress
         self._ws = None
@@ -369,6 +376,18 @@ class Server:
                 "solve_many_host_ms": round(1e3 * solve, 2),
                 "grouped_launches_gpu_ms": round(grouped, 2)}
 
+    @staticmethod
+    def step_cost(sp: JobSpec) -> float:
+        """Relative cost of one VMC step: sampling and E_loc ~ ns N^2 M (2N
+        proposals and ~N connected configurations per sample, O(M) each), the
+        Jacobian ns N M, the MinSR Gram ns^2 Np (the largest term here)."""
+        return float(sp.ns) * sp.N * sp.N * sp.M + float(sp.ns) ** 2 * sp.nparams
+
+    def paces(self, specs: Sequence[JobSpec]) -> List[int]:
+        """Steps per round of each job: floor(largest step cost / its own), in 1..pacing."""
+        cmax = max(self.step_cost(sp) for sp in specs)
+        return [max(1, min(self.pacing, int(cmax / self.step_cost(sp)))) for sp in specs]
+
     def run(self, specs: Sequence[JobSpec]) -> ServeReport:
         rep = ServeReport()
         pending = sorted(specs, key=lambda s: (s.arrival, s.job_id))
@@ -391,22 +410,27 @@ class Server:
                 time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
                 continue
             rep.max_active = max(rep.max_active, len(active))
-            self.round(active)
-            rep.rounds += 1
-            finished = [j for j in active if j.steps_done >= j.spec.nsteps]
-            if finished:
-                torch.cuda.synchronize()
-                tdone = time.perf_counter() - t0
-                for j in finished:
-                    rep.latencies.append(tdone - j.spec.arrival)
-                    rep.total_steps += j.spec.nsteps
-                    used -= footprint_bytes(j.spec, self.gram)
-                    if self.keep_results:
-                        rep.results[j.spec.job_id] = j.result()
-                    active.remove(j)
-                if self.progress:
-                    self.progress(f"{len(rep.latencies)} jobs done, {len(active)} active, {len(pending)} waiting, "
-                                  f"t = {tdone:.1f} s")
+            paces = self.paces([j.spec for j in active])
+            for r in range(max(paces)):
+                sub = [j for j, k in zip(active, paces) if k > r and j.steps_done < j.spec.nsteps]
+                if not sub:
+                    break
+                self.round(sub)
+                rep.rounds += 1
+                finished = [j for j in sub if j.steps_done >= j.spec.nsteps]
+                if finished:
+                    torch.cuda.synchronize()
+                    tdone = time.perf_counter() - t0
+                    for j in finished:
+                        rep.latencies.append(tdone - j.spec.arrival)
+                        rep.total_steps += j.spec.nsteps
+                        used -= footprint_bytes(j.spec, self.gram)
+                        if self.keep_results:
+                            rep.results[j.spec.job_id] = j.result()
+                    if self.progress:
+                        self.progress(f"{len(rep.latencies)} jobs done, {len(active) - len(finished)} active, "
+                                      f"{len(pending)} waiting, t = {tdone:.1f} s")
+            active = [j for j in active if j.steps_done < j.spec.nsteps]
         torch.cuda.synchronize()
         rep.makespan = time.perf_counter() - t0
         return rep

@@ -222,3 +222,61 @@ def test_snr_needs_its_workspace_flag():
                              mode | _lib.QTX_SOLVE_MINNORM | _lib.QTX_SOLVE_LSTSQ, 0, 0.0, 0.0, 1e-6, 0.0,
                              C.c_void_p(step.data_ptr()), C.c_void_p(ws.data_ptr()), ws.numel())
     assert rc == _lib.QTX_EINVAL
+
+
+@pytest.mark.parametrize("shape,solver", [
+    ((2500, 4000), "chol"), ((1500, 3000), "pf"), ((600, 1500), "syevd"), ((3000, 700), "snr")])
+def test_caller_scratch_matches_library_scratch(shape, solver):
+    """QTX_SOLVE_BLAS_WS: the rocBLAS / rocSOLVER scratch taken from the tail
+    of the caller's workspace (sized by qtx_minsr_blas_ws_bytes) gives the
+    same update as scratch rocBLAS allocates itself, on the blocked Cholesky
+    (n = 2500, three diagonal blocks), the partial fractions, dsyevd and the
+    SR branch with SNR truncation."""
+    ns, np_ = shape
+    O, e, _ = _problem(7, ns, np_, np.float32)
+    Od, mean, eb, _ = _device_inputs(O, e, None)
+    mk = {"chol": lambda: qtx.optimizer.auto_shift_eig(rshift=0.0, ashift=1e-3),
+          "pf": lambda: qtx.optimizer.auto_pinv_eig(rtol=1e-4),
+          "syevd": lambda: qtx.optimizer.auto_pinv_eig(rtol=1e-10),
+          "snr": lambda: qtx.optimizer.auto_pinv_eig(rtol=1e-8, tol_snr=1.0)}[solver]
+    s_lib, s_user = mk(), mk()
+    s_lib.caller_scratch = False
+    own = int(_lib.lib().qtx_minsr_ws_bytes(ns, np_, s_user.cmode, 0))
+    total = s_user.ws_bytes(ns, np_)
+    print(f"{shape} {solver}: own {own} B, rocBLAS/rocSOLVER scratch {total - own} B")
+    x_lib = qtx.optimizer.solve_step(Od, mean, eb, s_lib).cpu().numpy()
+    info_lib = qtx.optimizer.last_solve_info()
+    x_user = qtx.optimizer.solve_step(Od, mean, eb, s_user).cpu().numpy()
+    info_user = qtx.optimizer.last_solve_info()
+    assert info_lib["path"] == info_user["path"]
+    # rocBLAS may pick other kernels with a user-owned workspace: rounding only
+    err = _rel(x_user, x_lib)
+    print(f"  caller scratch against library scratch: {err:.2e}")
+    assert err < 1e-12, err
+
+
+def test_caller_scratch_too_small_is_refused():
+    """A workspace tail below the rocSOLVER need is a workspace error, not a
+    fault: dsyevd at n = 600 with a 256-byte tail (the smallest accepted;
+    without the tail the call is refused before anything runs)."""
+    ns, np_ = 600, 1500
+    O, e, _ = _problem(8, ns, np_, np.float64)
+    Od, mean, eb, _ = _device_inputs(O, e, None)
+    s = qtx.optimizer.auto_pinv_eig(rtol=1e-10)
+    lib = _lib.lib()
+    own = int(lib.qtx_minsr_ws_bytes(ns, np_, s.cmode, 0))
+    assert s.ws_bytes(ns, np_) > own  # dsyevd needs scratch
+    step = torch.empty(np_, dtype=torch.float64, device=DEV)
+    ws0 = torch.empty(own, dtype=torch.uint8, device=DEV)
+    rc0 = lib.qtx_minsr_solve(C.c_void_p(torch.cuda.current_stream().cuda_stream), _lib.QTX_F64,
+                              C.c_void_p(Od.data_ptr()), C.c_void_p(mean.data_ptr()), None, C.c_void_p(eb.data_ptr()),
+                              ns, np_, s.cmode, 0, 0.0, 0.0, 1e-10, 0.0, C.c_void_p(step.data_ptr()),
+                              C.c_void_p(ws0.data_ptr()), ws0.numel())
+    assert rc0 == _lib.QTX_EWORKSPACE
+    ws = torch.empty(own + 256, dtype=torch.uint8, device=DEV)
+    rc = lib.qtx_minsr_solve(C.c_void_p(torch.cuda.current_stream().cuda_stream), _lib.QTX_F64,
+                             C.c_void_p(Od.data_ptr()), C.c_void_p(mean.data_ptr()), None, C.c_void_p(eb.data_ptr()),
+                             ns, np_, s.cmode, 0, 0.0, 0.0, 1e-10, 0.0, C.c_void_p(step.data_ptr()),
+                             C.c_void_p(ws.data_ptr()), ws.numel())
+    torch.cuda.synchronize()
+    assert rc == _lib.QTX_EWORKSPACE, (rc, lib.qtx_last_error())

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from quantax_amd import _lib
-from quantax_amd.serve import JobSpec, admit, c5_stream
+from quantax_amd.serve import JobSpec, Server, admit, c5_stream
 
 
 def test_c5_stream_shape():
@@ -35,6 +35,21 @@ def test_admit_fifo_budget():
     pending = list(specs)
     got = admit(pending, now=1.5, used=0, budget=1000, cost=lambda s: cost[s.job_id])
     assert [s.job_id for s in got] == [0, 1]  # later jobs have not arrived
+
+
+def test_pacing_by_step_cost():
+    """Steps per round: the largest job takes one, a job costing 1/k of it
+    min(k, pacing), equal jobs one each."""
+    big = JobSpec(job_id=0, L=16, J2=0.5, alpha=4, ns=8192)
+    mid = JobSpec(job_id=1, L=12, J2=0.5, alpha=2, ns=4096)
+    small = JobSpec(job_id=2, L=6, J2=0.0, alpha=1, ns=1024)
+    srv = Server(hbm_budget=1)
+    c = [Server.step_cost(s) for s in (big, mid, small)]
+    assert c[0] > c[1] > c[2]
+    p = srv.paces([big, mid, small])
+    assert p[0] == 1 and p[2] == 8 and p[1] == min(8, int(c[0] / c[1]))
+    assert srv.paces([mid, mid]) == [1, 1]
+    assert Server(hbm_budget=1, pacing=1).paces([big, mid, small]) == [1, 1, 1]
 
 
 def test_group_entry_points_reject_bad_jobs():
