@@ -156,7 +156,10 @@ int qtx_rbm_jacobian(void* stream, int dtype, const void* packed, int32_t M, int
  * reference, which runs one lattice per process (sites/sites.py:46-51); the
  * per-job stages are the ones qtx_rbm_* replace (float32 parameters).
  * jobs: host array, read before the call returns (it is staged through a
- * pinned per-thread buffer).  ws: qtx_rbm_group_ws_bytes(njobs) bytes of
+ * pinned per-thread buffer, the one allocation these entry points make
+ * themselves: hipHostMalloc once per thread, grown to the largest job table
+ * seen and kept, 224 bytes per job, so the host-to-device copy of the table
+ * is asynchronous).  ws: qtx_rbm_group_ws_bytes(njobs) bytes of
  * device memory holding the launch's job table; calls that share ws must be
  * issued on one stream. */
 typedef struct qtx_rbm_job {
