@@ -25,11 +25,24 @@ Pinning (how each part is checked independently of retyping its formula):
                      against finite differences and direct products.
 * ``models.resconv_*`` — translation invariance, the LSE head closed form,
                      finite-difference and torch-autograd (float64) checks of
-                     the backward pass.
+                     the backward pass, for the tanh and erf GELU and the
+                     opt-in ReLU variant.
+* ``cpu_resconv``  — the FP32 torch-CPU forward (the CPU baseline) against the
+                     float64 restatement above, to FP32 rounding.
 * ``sampler``      — exact stationary distribution (chi^2 on N <= 8) and
                      the detailed structure of proposals.
 * ``sr``           — scipy cho_solve / eigh / lstsq minimum-norm solutions,
-                     the push-through identity SR == MinSR for a shift > 0.
+                     the push-through identity SR == MinSR for a shift > 0;
+                     ``sum_without_noise`` by columns of known statistics
+                     (constant, +-a, mu +- sigma: snr = |mu| sqrt(m) / sigma);
+                     the SNR-truncated minnorm and lstsq branches equal for
+                     any tol_snr (their components differ by the singular
+                     values, which the SNR does not see); ``minsr_pinv_eig``
+                     = T^-1 b on well-conditioned indefinite T and
+                     A^T minsr_pinv_eig(A A^T, b) = minnorm_pinv_eig(A, b).
+* ``sampler.reweight_factor`` — the closed form |psi|^(2-n) / mean (constant 1
+                     at n = 2); ``operator.expectation`` — the dense <H> and
+                     <H^2> - <H>^2 on N <= 12.
 
 Parity unpinned (no reference output exists for it, JAX is absent): the
 threefry streams of ``jax.random`` (replaced here by supplied Philox streams),
