@@ -349,7 +349,9 @@ __device__ __attribute__((always_inline)) inline void rc_forward(const RcGeom& g
                            const int8_t* sp, char* act, float* red, double* lp_out) {
   using K = RcK<C, MODE>;
   constexpr int CT = K::CT;
-  const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));  // per-lane addresses formed inside the forward
+  const int lane = tid % WAVE, wv = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
   const int gq = lane / 16;
   const int nitems = B * g.NT;
   const int64_t act_cfg_bytes = (int64_t)g.N * C * 4;
@@ -781,7 +783,12 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
   constexpr int ROWB = 16 * 64;
   constexpr int CFGB = L0 * ROWB;
   constexpr int KSW = (MODE == QTX_CONV_FP32) ? 36 : 6;
-  const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+  // the lane index through an opaque move: every per-lane address below is
+  // then computed inside the forward instead of being hoisted into
+  // registers held (or spilled) across a caller's whole loop
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const int lane = tid % WAVE, wv = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
   const int gq = lane / 16, x = lane % 16;
   const int b = wv / WPC, y0 = RPW * (wv % WPC);
   const float* pf = reinterpret_cast<const float*>(packed);
@@ -1385,6 +1392,10 @@ __global__ __launch_bounds__(W * 64, (ROWS == 2 ? (W == 4 * B ? 3 : 2) : (ROWS =
   // chain b is computed by wave B + b while wave b draws the move
   constexpr bool kThrWave = (2 * B <= W);
   for (int t = 0; t < nsteps; ++t) {
+    // the lane index through an opaque move, as in the forwards: the step's
+    // per-lane LDS addresses are formed here, not held across the loop
+    int lane = threadIdx.x % WAVE;
+    asm volatile("" : "+v"(lane));
     // proposals: one wave per chain
     for (int b = wv; b < B; b += W) {
       int64_t c = c0 + b;
