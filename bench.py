@@ -272,24 +272,35 @@ def precision_report(cfg, state, H, solver, last, torch, qtx, breakdown=False):
 
 def reference_default_solve(last, solver, torch, qtx):
     """The last step's solve with the reference's default cutoff rtol = 1e-12
-    (``optimizer/solver.py:12-21``): below 1e-5 lambda_max, so it takes the
-    dsyevd route; HIP events around the call, the step's own O and Ebar."""
+    (``optimizer/solver.py:12-21``), below 1e-5 lambda_max: the shifted
+    Cholesky route when T allows it, else dsyevd; and the same solve forced
+    through dsyevd (EIGH_PINV_DIRECT) for its time and the two results'
+    relative difference.  HIP events around each call, the step's own O and
+    Ebar, the faster of two calls."""
+    def timed(s):
+        ms, x = [], None
+        for _ in range(2):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            x = qtx.optimizer.solve_step(last["O"], last["mean"], last["ebar"], s)
+            b.record()
+            torch.cuda.synchronize()
+            ms.append(a.elapsed_time(b))
+        return min(ms), x, qtx.optimizer.last_solve_info()
+
     ref = qtx.optimizer.Solver(solver.mode, solver.rshift, solver.ashift, 1e-12, 0.0, solver.gram)
-    ms = []
-    for _ in range(2):
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record()
-        qtx.optimizer.solve_step(last["O"], last["mean"], last["ebar"], ref)
-        b.record()
-        torch.cuda.synchronize()
-        ms.append(a.elapsed_time(b))
-    info = qtx.optimizer.last_solve_info()
+    ms, x, info = timed(ref)
+    direct = qtx.optimizer.Solver(qtx._lib.QTX_SOLVE_EIGH_PINV_DIRECT, solver.rshift, solver.ashift, 1e-12, 0.0,
+                                  solver.gram)
+    ms_d, x_d, _ = timed(direct)
     torch.cuda.empty_cache()
-    rep = {"rtol": 1e-12, "gram": solver.gram, "ms": round(min(ms), 1), "path": info["path"],
-           "fallback": info["fallback"], "note": "not part of the timed step, which uses rtol = "
+    rep = {"rtol": 1e-12, "gram": solver.gram, "ms": round(ms, 1), "path": info["path"],
+           "fallback": info["fallback"], "dsyevd_ms": round(ms_d, 1),
+           "rel_diff_vs_dsyevd": (torch.linalg.norm(x - x_d) / torch.linalg.norm(x_d)).item(),
+           "note": "not part of the timed step, which uses rtol = "
            f"{solver.rtol:g} (partial-fraction route)"}
-    print(f"[bench] reference-default solve (rtol 1e-12): {rep['ms']} ms via {rep['path']}", file=sys.stderr,
-          flush=True)
+    print(f"[bench] reference-default solve (rtol 1e-12): {rep['ms']} ms via {rep['path']}; dsyevd "
+          f"{rep['dsyevd_ms']} ms, relative difference {rep['rel_diff_vs_dsyevd']:.2e}", file=sys.stderr, flush=True)
     return rep
 
 

@@ -286,7 +286,13 @@ int qtx_scale_columns(void* stream, int o_dtype, void* O, double* mean, int64_t 
  *                  shifted system solved by Cholesky of (T - Re p)^2 + (Im p)^2 (the
  *                  three factored together by this library's blocked Cholesky) plus
  *                  iterative refinement against T; lambda_max by Lanczos.  Smaller
- *                  cutoffs, or a failed factorisation, use rocSOLVER dsyevd.
+ *                  cutoffs (the reference default rtol = 1e-12): if T_d - 150 c I
+ *                  factors, T_d = T + lambda_max 1 1^T / ns for MinSR (the centred
+ *                  Gram's null direction 1 deflated; b projected off it), T_d = S for
+ *                  SR, then every other eigenvalue exceeds 150 c, where g(v) = 1/v to
+ *                  (c/v)^6 <= 8.8e-14, and T_d^-1 b is refined with that factor; a
+ *                  failed or non-converging factorisation (and reweighted MinSR)
+ *                  uses rocSOLVER dsyevd, as does a failed partial-fraction one.
  * mode EIGH_PINV_DIRECT: the same g(T) b through dsyevd always.
  * gram: QTX_GRAM_* (the ns >= np branch always uses FP64 products: Obar formed
  *       in FP64 by row blocks of at most 256 MiB, S accumulated by rocBLAS DGEMM).
@@ -366,13 +372,15 @@ int qtx_pinv_eig(void* stream, const double* T, int64_t n, const double* b, doub
                  double* x, void* ws, size_t ws_bytes);
 
 /* What the calling thread's last qtx_minsr_solve did: path 0 Cholesky shift,
- * 1 partial fractions, 2 dsyevd (-1: none or failed); Lanczos steps and the
- * lambda_max they gave (0 when not run); fallback (nullable): why EIGH_PINV
- * left the partial-fraction route for dsyevd: 0 it did not, 1 cutoff below
- * 1e-5 lambda_max, 2 non-finite Lanczos coefficients (non-finite T), 3
- * lambda_max <= 0, 4 a Cholesky factorisation failed, 5 SNR truncation (needs
- * the eigenvectors).  Diagnostic only: no counterpart in the reference, which
- * always runs eigh. */
+ * 1 partial fractions, 2 dsyevd, 3 shifted Cholesky (small cutoffs) (-1: none
+ * or failed); Lanczos steps and the lambda_max they gave (0 when not run);
+ * fallback (nullable): why EIGH_PINV took dsyevd: 0 it did not, 1 non-finite
+ * cutoff or a small cutoff with reweighted MinSR, 2 non-finite Lanczos
+ * coefficients (non-finite T), 3 lambda_max <= 0, 4 a partial-fraction
+ * Cholesky factorisation failed, 5 SNR truncation (needs the eigenvectors), 6
+ * the shifted factorisation failed (an eigenvalue of T_d at or below 150 c),
+ * 7 its refinement did not converge.  Diagnostic only: no counterpart in the
+ * reference, which always runs eigh. */
 int qtx_minsr_last_path(int* path, int* lanczos_steps, double* lambda_max, int* fallback);
 
 /* theta <- theta - lr * step in dtype, skipped entirely when any step entry is

@@ -697,6 +697,116 @@ __global__ __launch_bounds__(256) void form_quad_kernel(const double* T2,
   }
 }
 
+// Shifted Cholesky route for small cutoffs (c < 1e-5 lambda_max, the
+// reference default rtol = 1e-12): G = T + (mu / n) 1 1^T - sigma I, lower
+// triangle (the upper part zero), factored once; x = T^+ b' is refined against
+// T + (mu / n) 1 1^T, which deflates the null direction 1 of the centred MinSR
+// Gram (mu = 0 for the SR branch), by conjugate gradients preconditioned
+// with that factor.
+__global__ __launch_bounds__(256) void form_shifted_kernel(const double* __restrict__ T, int64_t n, double mu_n,
+                                                           double sigma, double* __restrict__ G) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = e / n, i = e - j * n;
+    G[e] = (i < j) ? 0.0 : T[e] + mu_n - (i == j ? sigma : 0.0);
+  }
+}
+
+__device__ double block_sum_1024(double v, double* red) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+  __syncthreads();
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < nw; ++w) s += red[w];
+  return s;
+}
+
+// single block: b' = b - mean(b) 1 (deflate) or b; x = 0; r = z = b'
+__global__ __launch_bounds__(1024) void shifted_init_kernel(const double* __restrict__ b, int64_t n, int deflate,
+                                                            double* __restrict__ bp, double* __restrict__ x,
+                                                            double* __restrict__ r, double* __restrict__ z) {
+  __shared__ double red[16];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += b[i];
+  const double mean = deflate ? block_sum_1024(s, red) / (double)n : 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double v = b[i] - mean;
+    bp[i] = v;
+    r[i] = v;
+    z[i] = v;
+    x[i] = 0.0;
+  }
+}
+
+// Conjugate gradients on T_d x = b' preconditioned by the factor of
+// G = T_d - sigma I (the preconditioned operator is I + sigma G^-1: eigenvalues
+// 1 + sigma / (v - sigma), clustered at 1 except for the few v near sigma).
+// s[0] holds r.z.  Single-block steps around the library's triangular solves.
+// step 1: q = T p (given) + (mu / n) sum(p); alpha = rz / p.q; x += alpha p;
+// r -= alpha q; z = r (then z <- G^-1 z by two triangular solves)
+__global__ __launch_bounds__(1024) void pcg_step1_kernel(double* __restrict__ q, const double* __restrict__ p,
+                                                         int64_t n, double mu_n, double* __restrict__ x,
+                                                         double* __restrict__ r, double* __restrict__ z,
+                                                         const double* __restrict__ s) {
+  __shared__ double red[16];
+  double sp = 0.0;
+  if (mu_n != 0.0)
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) sp += p[i];
+  const double add = (mu_n != 0.0) ? mu_n * block_sum_1024(sp, red) : 0.0;
+  double pq = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double qi = q[i] + add;
+    q[i] = qi;
+    pq += p[i] * qi;
+  }
+  pq = block_sum_1024(pq, red);
+  const double alpha = (pq > 0.0 && std::isfinite(pq)) ? s[0] / pq : 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    z[i] = ri;
+  }
+}
+
+// step 2 (first = 1: p = z): rz' = r.z; beta = rz' / rz; p = z + beta p; s[0] = rz'
+__global__ __launch_bounds__(1024) void pcg_step2_kernel(const double* __restrict__ r, const double* __restrict__ z,
+                                                         int64_t n, int first, double* __restrict__ p,
+                                                         double* __restrict__ s) {
+  __shared__ double red[16];
+  double rz = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) rz += r[i] * z[i];
+  rz = block_sum_1024(rz, red);
+  const double beta = (!first && s[0] > 0.0 && std::isfinite(rz)) ? rz / s[0] : 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = z[i] + beta * p[i];
+  __syncthreads();
+  if (threadIdx.x == 0) s[0] = rz;
+}
+
+// the true residual of the result: norms <- |b' - T_d x|^2, |x|^2 (Tx given)
+__global__ __launch_bounds__(1024) void pcg_check_kernel(const double* __restrict__ bp, const double* __restrict__ Tx,
+                                                         const double* __restrict__ x, int64_t n, double mu_n,
+                                                         double* __restrict__ norms) {
+  __shared__ double red[16];
+  double sx = 0.0;
+  if (mu_n != 0.0)
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) sx += x[i];
+  const double add = (mu_n != 0.0) ? mu_n * block_sum_1024(sx, red) : 0.0;
+  double rr = 0.0, xx = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double ri = bp[i] - Tx[i] - add;
+    rr += ri * ri;
+    xx += x[i] * x[i];
+  }
+  rr = block_sum_1024(rr, red);
+  xx = block_sum_1024(xx, red);
+  if (threadIdx.x == 0) {
+    norms[0] = rr;
+    norms[1] = xx;
+  }
+}
+
 // R = [b, 0], Z = 0   (R, Z: n x 2 column-major)
 __global__ void refine_init_kernel(const double* __restrict__ rhs, int64_t n, double* __restrict__ R,
                                    double* __restrict__ Z) {
@@ -875,6 +985,14 @@ static inline bool use_minnorm(int64_t ns, int64_t np, int mode) {
   return ns < np;
 }
 constexpr double kSnrMin = 1e-6;  // tol_snr at or below it: no truncation (solver.py:124)
+// shifted Cholesky route: G = T_d - kShiftCut c I must factor, which puts every
+// eigenvalue of T_d above 150 c, where the soft inverse differs from 1/v by
+// (c/v)^6 <= 8.8e-14; kShiftIters preconditioned CG steps, accepted when the
+// true residual's normwise backward error |b' - T_d x| / (lambda_max |x|) is
+// below kShiftAccept (the accuracy a backward-stable eigensolver attains)
+constexpr double kShiftCut = 150.0;
+constexpr int kShiftIters = 40;
+constexpr double kShiftAccept = 1e-15;
 
 struct SolveWs {
   double* T;       // n x n
@@ -1348,6 +1466,7 @@ struct Task {
   bool failed = false;
   bool need_syevd = false;  // phase 2 runs the dsyevd route
   bool pf = false;          // partial-fraction factorisations enqueued
+  bool cf = false;          // shifted Cholesky route enqueued (small cutoffs)
   int mcap = 0, done = 0;   // Lanczos steps available / enqueued
   double lmax = 0.0;
   int lanczos = 0;
@@ -1428,11 +1547,6 @@ static int phase1(Task& t, rocblas_handle h) {
     t.need_syevd = true;
     return QTX_OK;
   }
-  if (io.atol == 0.0 && !(io.rtol >= 1e-5)) {  // cutoff too small for the refinement to contract
-    t.fallback = 1;
-    t.need_syevd = true;
-    return QTX_OK;
-  }
   // lambda_max = max |eigenvalue| of the PSD T by Lanczos (three-term
   // recurrence; the largest Ritz value is checked every 32 steps)
   hipLaunchKernelGGL(lanczos_init_kernel, dim3(1), dim3(1024), 0, t.st, t.n, t.w.V, t.w.ab + kLanczosMax);
@@ -1491,11 +1605,63 @@ static int phase2(Task& t, rocblas_handle h) {
       t.fallback = 2;
     } else if (!(lmax > 0.0) || !std::isfinite(lmax)) {
       t.fallback = 3;
-    } else if (!(rel >= 1e-5) || !std::isfinite(c)) {
-      t.fallback = 1;
+    } else if (!std::isfinite(c) || (!(rel >= 1e-5) && t.minnorm && io.r != nullptr)) {
+      t.fallback = 1;  // (reweighted MinSR: the null direction of T is not 1)
     }
     if (t.fallback) {
       t.need_syevd = true;
+    } else if (!(rel >= 1e-5)) {
+      // small cutoff: T_d x = b' by the shifted factorisation and refinement
+      const rocblas_int ni = (rocblas_int)t.n;
+      const double one = 1.0;
+      const double mu_n = t.minnorm ? lmax / (double)t.n : 0.0;
+      const int64_t nn = t.n * t.n;
+      const unsigned gq = (unsigned)std::min<int64_t>((nn + 255) / 256, kGridCap);
+      const unsigned gm = (unsigned)((t.n + 3) / 4);
+      hipLaunchKernelGGL(fill_i32_kernel, dim3(nblk(3 * kInfoPerFactor, 256)), dim3(256), 0, t.st, t.w.info,
+                         (int64_t)(3 * kInfoPerFactor), 0);
+      hipLaunchKernelGGL(form_shifted_kernel, dim3(gq), dim3(256), 0, t.st, t.w.T, t.n, mu_n, kShiftCut * c, t.w.T2);
+      QTX_LAUNCH_CHECK();
+      int rc = blocked_potrf(h, t.st, t.n, t.w.T2, nn, 1, t.w.info, kInfoPerFactor, t.w.dtile);
+      if (rc) return rc;
+      double* r = t.w.R;
+      double* z = t.w.R + t.n;
+      double* p = t.w.R + 2 * t.n;
+      double* q = t.w.TW;
+      double* s = t.w.scalars + 4;
+      hipLaunchKernelGGL(shifted_init_kernel, dim3(1), dim3(1024), 0, t.st, t.w.y, t.n, t.minnorm ? 1 : 0, t.w.z,
+                         t.w.x, r, z);
+      QTX_LAUNCH_CHECK();
+      // L^-1 in place of L: each preconditioner application is then two
+      // bandwidth-bound triangular products instead of two triangular solves
+      // (a single-vector dtrsm at n = 16384 ran at ~10 ms)
+      if ((rc = rb(rocsolver_dtrtri(h, rocblas_fill_lower, rocblas_diagonal_non_unit, ni, t.w.T2, ni,
+                                    t.w.info + kInfoPerFactor),
+                   "rocsolver_dtrtri")))
+        return rc;
+      auto precondition = [&]() -> int {  // z <- G^-1 z = L^-T L^-1 z
+        int rc2 = rb(rocblas_dtrmv(h, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit, ni,
+                                   t.w.T2, ni, z, 1),
+                     "rocblas_dtrmv");
+        if (rc2) return rc2;
+        return rb(rocblas_dtrmv(h, rocblas_fill_lower, rocblas_operation_transpose, rocblas_diagonal_non_unit, ni,
+                                t.w.T2, ni, z, 1),
+                  "rocblas_dtrmv");
+      };
+      if ((rc = precondition())) return rc;
+      hipLaunchKernelGGL(pcg_step2_kernel, dim3(1), dim3(1024), 0, t.st, r, z, t.n, 1, p, s);
+      for (int it = 0; it < kShiftIters; ++it) {
+        hipLaunchKernelGGL(matvec_rows_kernel<1>, dim3(gm), dim3(256), 0, t.st, t.w.T, t.n, p, q);
+        hipLaunchKernelGGL(pcg_step1_kernel, dim3(1), dim3(1024), 0, t.st, q, p, t.n, mu_n, t.w.x, r, z, s);
+        QTX_LAUNCH_CHECK();
+        if ((rc = precondition())) return rc;
+        hipLaunchKernelGGL(pcg_step2_kernel, dim3(1), dim3(1024), 0, t.st, r, z, t.n, 0, p, s);
+      }
+      hipLaunchKernelGGL(matvec_rows_kernel<1>, dim3(gm), dim3(256), 0, t.st, t.w.T, t.n, t.w.x, q);
+      hipLaunchKernelGGL(pcg_check_kernel, dim3(1), dim3(1024), 0, t.st, t.w.z, q, t.w.x, t.n, mu_n,
+                         t.w.scalars + 6);
+      QTX_LAUNCH_CHECK();
+      t.cf = true;
     } else {
       // refinement contracts by about 4 u / rel^2 per pass (4.4e-8 at rel = 1e-4,
       // 4.4e-6 at 1e-5): two passes reach ~1e-15 for rel >= 1e-4, three for 1e-5
@@ -1598,6 +1764,28 @@ static int phase3(Task& t, rocblas_handle h) {
       t.path = 1;
     }
   }
+  if (t.cf) {
+    std::vector<int32_t> info_h(kInfoPerFactor + 1);  // the factorisation's blocks, then dtrtri
+    double nrm[2];
+    QTX_HIP(hipMemcpyAsync(info_h.data(), t.w.info, info_h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, t.st));
+    QTX_HIP(hipMemcpyAsync(nrm, t.w.scalars + 6, sizeof(nrm), hipMemcpyDeviceToHost, t.st));
+    QTX_HIP(hipStreamSynchronize(t.st));
+    bool ok = true;
+    for (int32_t v : info_h) ok = ok && (v == 0);
+    // nrm: |b' - T_d x|^2, |x|^2 of the result
+    const double res = sqrt(nrm[0]), xn = sqrt(nrm[1]);
+    const bool converged = std::isfinite(res) && std::isfinite(xn) && res <= kShiftAccept * t.lmax * xn;
+    if (ok && converged) {
+      hipLaunchKernelGGL(copy_f64_kernel, dim3(nblk(t.n, 256)), dim3(256), 0, t.st, t.w.x, t.w.y, t.n);
+      QTX_LAUNCH_CHECK();
+      t.path = 3;
+    } else {  // T and y are intact: take the dsyevd route
+      t.fallback = ok ? 7 : 6;
+      int rc = snr_syevd(t, h);
+      if (rc < 0) return rc;
+      t.path = 2;
+    }
+  }
   if (t.minnorm) {
     int rc = launch_obar_t_y<TI>(t.st, (const TI*)io.O, io.mean, io.r, t.w.y, io.ns, io.np, io.step);
     if (rc) return rc;
@@ -1671,6 +1859,11 @@ static int blas_query(rocblas_handle h, int64_t ns, int64_t np, int mode, SolveW
     good &= ok(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, ni, w.T, ni, w.D, w.E, w.info));
     good &= ok(rocblas_dgemv(h, rocblas_operation_transpose, ni, ni, &one, w.T, ni, w.y, 1, &zero, w.z, 1));
     good &= ok(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &one, w.T, ni, w.z, 1, &zero, w.y, 1));
+  }
+  if (bm == QTX_SOLVE_EIGH_PINV) {  // the small-cutoff route: inverse factor and its products
+    good &= ok(rocsolver_dtrtri(h, rocblas_fill_lower, rocblas_diagonal_non_unit, ni, w.T2, ni, w.info));
+    for (rocblas_operation op : {rocblas_operation_none, rocblas_operation_transpose})
+      good &= ok(rocblas_dtrmv(h, rocblas_fill_lower, op, rocblas_diagonal_non_unit, ni, w.T2, ni, w.R, 1));
   }
   if (bm == QTX_SOLVE_EIGH_PINV) {  // partial fractions: T^2 and the refinement's triangular solves
     good &= ok(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, ni, ni, &one, w.T, ni, &zero, w.T2, ni));

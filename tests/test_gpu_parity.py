@@ -386,11 +386,31 @@ def test_chol_shift_indefinite_reports_block(rng):
         qtx.optimizer.solve_step(Od, mean, eb, qtx.optimizer.auto_shift_eig(rshift=0.0, ashift=-1e3))
 
 
-def test_pinv_small_cutoff_uses_syevd(rng):
-    O, E = _solver_case(128, 500, np.float32, rng)
-    Od, mean, eb = _solve_inputs(O, E)
-    qtx.optimizer.solve_step(Od, mean, eb, qtx.optimizer.auto_pinv_eig(rtol=1e-8))
-    assert qtx.optimizer.last_solve_info()["path"] == "syevd"
+def test_pinv_small_cutoff_routes(rng):
+    """Cutoffs below 1e-5 lambda_max (the reference default rtol = 1e-12):
+    a well-conditioned T takes the shifted Cholesky route and equals the
+    eigh soft inverse; duplicate samples (a second null direction of T, an
+    eigenvalue below 150 c) make the shifted factorisation fail and the solve
+    takes dsyevd; both against the oracle."""
+    for dup in (False, True):
+        O, E = _solver_case(128, 500, np.float32, rng)
+        if dup:
+            O[7] = O[3]
+            E[7] = E[3]
+        Od, mean, eb = _solve_inputs(O, E)
+        for rtol in (1e-8, 1e-12):
+            step = qtx.optimizer.solve_step(Od, mean, eb, qtx.optimizer.auto_pinv_eig(rtol=rtol)).cpu().numpy()
+            info = qtx.optimizer.last_solve_info()
+            Obar, _ = osr.get_Obar(O.astype(np.float64))
+            Ebar, _, _ = osr.get_Ebar(E)
+            ref = osr.auto_pinv_eig(Obar, Ebar, rtol=rtol)
+            err = np.linalg.norm(step - ref) / np.linalg.norm(ref)
+            print(f"duplicates {dup} rtol {rtol}: {info['path']} / {info['fallback']}, err {err:.2e}")
+            if dup:
+                assert info["path"] == "syevd" and info["fallback"] == "shifted_cholesky_failed", info
+            else:
+                assert info["path"] == "shifted_cholesky", info
+            assert err < 1e-9, err
 
 
 def test_pinv_zero_gram(rng):

@@ -81,8 +81,12 @@ def test_forced_branches_match_oracle(shape, odt):
          osr.lstsq_shift_eig(A, b, rshift=0.0, ashift=1e-3), "chol_shift"),
         (qtx.optimizer.minnorm_pinv_eig(rtol=1e-4), osr.minnorm_pinv_eig(A, b, rtol=1e-4), "partial_fractions"),
         (qtx.optimizer.lstsq_pinv_eig(rtol=1e-4), osr.lstsq_pinv_eig(A, b, rtol=1e-4), "partial_fractions"),
-        (qtx.optimizer.minnorm_pinv_eig(rtol=1e-10), osr.minnorm_pinv_eig(A, b, rtol=1e-10), "syevd"),
-        (qtx.optimizer.lstsq_pinv_eig(rtol=1e-10), osr.lstsq_pinv_eig(A, b, rtol=1e-10), "syevd"),
+        # small cutoff: the shifted Cholesky route on the shape's own branch;
+        # the other branch's Gram is rank-deficient, so dsyevd
+        (qtx.optimizer.minnorm_pinv_eig(rtol=1e-10), osr.minnorm_pinv_eig(A, b, rtol=1e-10),
+         "shifted_cholesky" if ns < np_ else "syevd"),
+        (qtx.optimizer.lstsq_pinv_eig(rtol=1e-10), osr.lstsq_pinv_eig(A, b, rtol=1e-10),
+         "syevd" if ns < np_ else "shifted_cholesky"),
     ]
     for solver, ref, path in cases:
         x = qtx.optimizer.solve_step(Od, mean, eb, solver).cpu().numpy()
@@ -225,21 +229,26 @@ def test_snr_needs_its_workspace_flag():
 
 
 @pytest.mark.parametrize("shape,solver", [
-    ((2500, 4000), "chol"), ((1500, 3000), "pf"), ((600, 1500), "syevd"), ((3000, 700), "snr")])
+    ((2500, 4000), "chol"), ((1500, 3000), "pf"), ((600, 1500), "syevd"), ((3000, 700), "snr"),
+    ((1200, 3000), "small")])
 def test_caller_scratch_matches_library_scratch(shape, solver):
     """QTX_SOLVE_BLAS_WS: the rocBLAS / rocSOLVER scratch taken from the tail
     of the caller's workspace (sized by qtx_minsr_blas_ws_bytes) gives the
     same update as scratch rocBLAS allocates itself, on the blocked Cholesky
-    (n = 2500, three diagonal blocks), the partial fractions, dsyevd and the
-    SR branch with SNR truncation."""
+    (n = 2500, three diagonal blocks), the partial fractions, dsyevd, the
+    SR branch with SNR truncation and the shifted Cholesky route of the
+    reference default cutoff (n = 1200, two diagonal blocks)."""
     ns, np_ = shape
     O, e, _ = _problem(7, ns, np_, np.float32)
     Od, mean, eb, _ = _device_inputs(O, e, None)
     mk = {"chol": lambda: qtx.optimizer.auto_shift_eig(rshift=0.0, ashift=1e-3),
           "pf": lambda: qtx.optimizer.auto_pinv_eig(rtol=1e-4),
           "syevd": lambda: qtx.optimizer.auto_pinv_eig(rtol=1e-10),
-          "snr": lambda: qtx.optimizer.auto_pinv_eig(rtol=1e-8, tol_snr=1.0)}[solver]
+          "snr": lambda: qtx.optimizer.auto_pinv_eig(rtol=1e-8, tol_snr=1.0),
+          "small": lambda: qtx.optimizer.auto_pinv_eig()}[solver]
     s_lib, s_user = mk(), mk()
+    if solver == "syevd":
+        s_lib.mode = s_user.mode = _lib.QTX_SOLVE_EIGH_PINV_DIRECT
     s_lib.caller_scratch = False
     own = int(_lib.lib().qtx_minsr_ws_bytes(ns, np_, s_user.cmode, 0))
     total = s_user.ws_bytes(ns, np_)
@@ -263,6 +272,7 @@ def test_caller_scratch_too_small_is_refused():
     O, e, _ = _problem(8, ns, np_, np.float64)
     Od, mean, eb, _ = _device_inputs(O, e, None)
     s = qtx.optimizer.auto_pinv_eig(rtol=1e-10)
+    s.mode = _lib.QTX_SOLVE_EIGH_PINV_DIRECT  # dsyevd always
     lib = _lib.lib()
     own = int(lib.qtx_minsr_ws_bytes(ns, np_, s.cmode, 0))
     assert s.ws_bytes(ns, np_) > own  # dsyevd needs scratch
