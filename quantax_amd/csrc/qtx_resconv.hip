@@ -133,7 +133,10 @@ __global__ void rc_pack_frag_kernel(const float* __restrict__ theta, int nb, cha
   int64_t w0, b0;
   rc_theta_offsets(nb, C, l, w0, b0);
   const int co = 16 * ct + lane % 16, g = lane / 16;
-  auto W = [&](int ci, int tap) -> float { return theta[w0 + ((int64_t)co * C + ci) * 9 + tap]; };
+  // window layout, split-bf16: conv1 layers l >= 2 carry the scale of the
+  // conv2 before them, which then outputs act(s x) / s
+  const float wsc = (WIN && MODE == QTX_CONV_BF16X3 && l >= 2 && (l % 2) == 0) ? 1.0f / sqrtf((float)(l / 2 + 1)) : 1.0f;
+  auto W = [&](int ci, int tap) -> float { return theta[w0 + ((int64_t)co * C + ci) * 9 + tap] * wsc; };
   char* dst = packed + off_frag + (int64_t)l * frag_bytes;
   if constexpr (MODE == QTX_CONV_FP32) {
     // ks = (tap, h, j'): ci = 16 h + 4 g + j'
@@ -239,7 +242,8 @@ __device__ __forceinline__ float act_grad_f(float x, int actk) {
 typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void split_bf16x4(v4f v, v4bf& hi, v4bf& lo) {
   const v2bf h01 = {(__bf16)v[0], (__bf16)v[1]}, h23 = {(__bf16)v[2], (__bf16)v[3]};
-  const uint32_t u01 = __builtin_bit_cast(uint32_t, h01), u23 = __builtin_bit_cast(uint32_t, h23);
+  uint32_t u01 = __builtin_bit_cast(uint32_t, h01), u23 = __builtin_bit_cast(uint32_t, h23);
+  asm("" : "+v"(u01), "+v"(u23));
   const float f0 = __builtin_bit_cast(float, u01 << 16), f1 = __builtin_bit_cast(float, u01 & 0xffff0000u);
   const float f2 = __builtin_bit_cast(float, u23 << 16), f3 = __builtin_bit_cast(float, u23 & 0xffff0000u);
   const v2bf l01 = {(__bf16)(v[0] - f0), (__bf16)(v[1] - f1)}, l23 = {(__bf16)(v[2] - f2), (__bf16)(v[3] - f3)};
@@ -746,7 +750,13 @@ __device__ __attribute__((always_inline)) inline void rc_forward_rows(const RcGe
 // (dy, 0) + (dy, 1), pr 1 = tap (dy, 2) in both K halves, whose split
 // products over the three dy take five instructions per output row, see the
 // packing; FP32 K-steps = taps), so each row is read from LDS once per wave
-// and layer instead of three times.
+// and layer instead of three times.  The hi-times-X-lo halves of tap (dy, 2)
+// pair rows y - 1 and y in one instruction: the lo fragment of row r is
+// loaded with its upper K half (lanes 32-63) from row r + 1, so
+// [X lo (y - 1); X lo (y)] is row y - 1's fragment as loaded, with no lane
+// select per row.  With split-bf16, a conv1 layer's packed weights carry the
+// scale s of the conv2 before it, whose epilogue then writes act(s x) / s
+// (s folds into the activation's constants: one multiply per value less).
 //
 // Issue structure: the activation and the conv kind (conv1 / conv2 with
 // residual) are compile-time inside a layer (one branch per layer), and the
@@ -772,6 +782,15 @@ typedef __attribute__((address_space(3))) void rc_lds_void;
 template <int ACT>
 __device__ __forceinline__ float act_t(float x) {
   return act_f(x, ACT);
+}
+
+// act(s x) / s for s > 0, with the layer constants k0 = s A, k1 = s^3 B (tanh
+// GELU) or k0 = s / sqrt(2) (erf GELU)
+template <int ACT>
+__device__ __forceinline__ float act_over_s(float x, float k0, float k1) {
+  if constexpr (ACT == QTX_ACT_RELU) return fmaxf(x, 0.0f);
+  else if constexpr (ACT == QTX_ACT_GELU_ERF) return 0.5f * x * (1.0f + erff(x * k0));
+  else return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(x * fmaf(x * x, k1, k0)));
 }
 
 template <int MODE, int B, int WPC>
@@ -832,7 +851,9 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
     v8bf h[(MODE == QTX_CONV_FP32) ? 1 : 2];
     v8bf l[(MODE == QTX_CONV_FP32) ? 1 : 2];
   };
-  auto load_row = [&](const char* cfg, int r) {
+  const int klo1n = klo[1] + (lane >= 32 ? ROWB : 0);
+  // nxt: the upper K half of F.l[1] comes from row r + 1 (no wrap), else row r
+  auto load_row = [&](const char* cfg, int r, bool nxt) {
     Frag F;
     const char* rb = cfg + ((r + L0) & (L0 - 1)) * ROWB;
     if constexpr (MODE == QTX_CONV_FP32) {
@@ -840,10 +861,9 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
       for (int dx = 0; dx < 3; ++dx) F.f[dx] = *reinterpret_cast<const v4f*>(rb + xoff[dx]);
     } else {
 #pragma unroll
-      for (int pr = 0; pr < 2; ++pr) {
-        F.h[pr] = *reinterpret_cast<const v8bf*>(rb + khi[pr]);
-        F.l[pr] = *reinterpret_cast<const v8bf*>(rb + klo[pr]);
-      }
+      for (int pr = 0; pr < 2; ++pr) F.h[pr] = *reinterpret_cast<const v8bf*>(rb + khi[pr]);
+      F.l[0] = *reinterpret_cast<const v8bf*>(rb + klo[0]);
+      F.l[1] = *reinterpret_cast<const v8bf*>(rb + (nxt ? klo1n : klo[1]));
     }
     return F;
   };
@@ -899,9 +919,14 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
     // rows y0 - 1 and y0 + RPW belong to the configuration's other waves:
     // read before the barrier below, after which every wave overwrites its
     // own rows in place
-    Frag Fm = load_row(in, y0 - 1);
-    Frag F0 = load_row(in, y0);
-    Frag Fe = load_row(in, y0 + RPW);
+    Frag Fm = load_row(in, y0 - 1, false);
+    if constexpr (MODE != QTX_CONV_FP32) {
+      // [X lo (y0 - 1); X lo (y0)]: row y0 - 1 may wrap
+      const char* pm = (lane < 32) ? in + ((y0 - 1 + L0) & (L0 - 1)) * ROWB : in + y0 * ROWB;
+      Fm.l[1] = *reinterpret_cast<const v8bf*>(pm + klo[1]);
+    }
+    Frag F0 = load_row(in, y0, true);
+    Frag Fe = load_row(in, y0 + RPW, false);
     float wf[(MODE == QTX_CONV_FP32) ? KSW : 1];
     v8bf wh[(MODE == QTX_CONV_FP32) ? 1 : KSW];
     v8bf wl[(MODE == QTX_CONV_FP32) ? 1 : KSW];
@@ -929,7 +954,7 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
       v4f acc = bias4;
       Frag Fp;
       if (i < RPW) {
-        Fp = (i + 1 < RPW) ? load_row(in, y0 + i + 1) : Fe;
+        Fp = (i + 1 < RPW) ? load_row(in, y0 + i + 1, i + 2 < RPW) : Fe;
         if constexpr (MODE == QTX_CONV_FP32) {
 #pragma unroll
           for (int dy = 0; dy < 3; ++dy) {
@@ -952,12 +977,7 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[2 * dy], R.h[0], acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[2 * dy + 1], R.h[1], acc, 0, 0, 0);
           }
-          const v4i m4 = __builtin_bit_cast(v4i, Fm.l[1]), z4 = __builtin_bit_cast(v4i, F0.l[1]);
-          v4i s4;
-#pragma unroll
-          for (int q = 0; q < 4; ++q) s4[q] = (lane < 32) ? m4[q] : z4[q];
-          const v8bf lo01 = __builtin_bit_cast(v8bf, s4);
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[1], lo01, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[1], Fm.l[1], acc, 0, 0, 0);
           acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[3], Fp.l[1], acc, 0, 0, 0);
         }
       }
@@ -971,44 +991,43 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
   };
 
   char* const buf = act + b * CFGB;  // one activation buffer per configuration, updated in place
-  for (int l = 1; l < 2 * nb - 1; ++l) {
-    const char* in = buf;
-    char* out = buf;
-    const bool is_conv2 = (l % 2) == 1;
-    const float scale = is_conv2 ? rsqrtf((float)(l / 2 + 2)) : 1.0f;
-    auto layer = [&](auto act_c, auto c2_c) {
-      constexpr int ACT = decltype(act_c)::value;
-      constexpr bool C2 = decltype(c2_c)::value;
-      run_layer(l, in, [&](int i, v4f acc) {
+  auto layers = [&](auto act_c) {
+    constexpr int ACT = decltype(act_c)::value;
+    // layers 1 .. 2 nb - 2 in pairs: conv2 of block l / 2 (residual), conv1 of block l / 2 + 1
+    for (int l = 1; l < 2 * nb - 1; l += 2) {
+      const float scale = rsqrtf((float)(l / 2 + 2));
+      constexpr float GA = -2.0f * 0.79788456080286536f * 1.4426950408889634f;
+      const float k0 = (ACT == QTX_ACT_GELU_ERF) ? scale * 0.70710678118654752f : scale * GA;
+      const float k1 = scale * scale * scale * (GA * 0.044715f);
+      run_layer(l, buf, [&](int i, v4f acc) {
         v4f o;
-        if constexpr (C2) {
-          v4f xr = acc + resr[i];
-          resr[i] = xr;
+        v4f xr = acc + resr[i];
+        resr[i] = xr;
+        if constexpr (MODE == QTX_CONV_FP32) {
 #pragma unroll
           for (int q = 0; q < 4; ++q) o[q] = act_t<ACT>(xr[q] * scale);
         } else {
+          // the next layer's weights carry the scale
 #pragma unroll
-          for (int q = 0; q < 4; ++q) o[q] = act_t<ACT>(acc[q]);
+          for (int q = 0; q < 4; ++q) o[q] = act_over_s<ACT>(xr[q], k0, k1);
         }
-        store4(out + (y0 + i) * ROWB, o);
+        store4(buf + (y0 + i) * ROWB, o);
       });
-    };
-    using A0 = std::integral_constant<int, QTX_ACT_GELU_TANH>;
-    using A1 = std::integral_constant<int, QTX_ACT_GELU_ERF>;
-    using A2 = std::integral_constant<int, QTX_ACT_RELU>;
-    if (actk == QTX_ACT_GELU_ERF) {
-      if (is_conv2) layer(A1{}, std::true_type{});
-      else layer(A1{}, std::false_type{});
-    } else if (actk == QTX_ACT_RELU) {
-      if (is_conv2) layer(A2{}, std::true_type{});
-      else layer(A2{}, std::false_type{});
-    } else {
-      if (is_conv2) layer(A0{}, std::true_type{});
-      else layer(A0{}, std::false_type{});
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      __syncthreads();
+      run_layer(l + 1, buf, [&](int i, v4f acc) {
+        v4f o;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) o[q] = act_t<ACT>(acc[q]);
+        store4(buf + (y0 + i) * ROWB, o);
+      });
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      __syncthreads();
     }
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
-  }
+  };
+  if (actk == QTX_ACT_GELU_ERF) layers(std::integral_constant<int, QTX_ACT_GELU_ERF>{});
+  else if (actk == QTX_ACT_RELU) layers(std::integral_constant<int, QTX_ACT_RELU>{});
+  else layers(std::integral_constant<int, QTX_ACT_GELU_TANH>{});
   // ---- last layer (conv2 of the last block, no bias, no GELU) and the head
   {
     const int l = 2 * nb - 1;
