@@ -237,16 +237,23 @@ __device__ __forceinline__ float act_grad_f(float x, int actk) {
 }
 
 // v = hi + lo + O(2^-16 |v|): hi = bf16(v) (round to nearest even), lo =
-// bf16(v - hi).  hi is converted once per pair and widened back to FP32 by
-// bit moves (bf16 is the top half of an FP32 word).
+// bf16(v - hi).  v - hi is formed by v_dot2c_f32_bf16 from the packed hi
+// pair, (hi0, hi1) . (-1, 0) + v0 and (hi0, hi1) . (0, -1) + v1, exact
+// (bf16 products are exact in FP32 and v - hi is representable), so hi is
+// never widened back to FP32 by bit moves.
 typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void split_bf16x4(v4f v, v4bf& hi, v4bf& lo) {
   const v2bf h01 = {(__bf16)v[0], (__bf16)v[1]}, h23 = {(__bf16)v[2], (__bf16)v[3]};
-  uint32_t u01 = __builtin_bit_cast(uint32_t, h01), u23 = __builtin_bit_cast(uint32_t, h23);
-  asm("" : "+v"(u01), "+v"(u23));
-  const float f0 = __builtin_bit_cast(float, u01 << 16), f1 = __builtin_bit_cast(float, u01 & 0xffff0000u);
-  const float f2 = __builtin_bit_cast(float, u23 << 16), f3 = __builtin_bit_cast(float, u23 & 0xffff0000u);
-  const v2bf l01 = {(__bf16)(v[0] - f0), (__bf16)(v[1] - f1)}, l23 = {(__bf16)(v[2] - f2), (__bf16)(v[3] - f3)};
+  // the selectors come from registers: an inline constant here (the
+  // compiler encodes e0 as "-1.0") is not read as (bf16 -1, 0) by the hardware
+  uint32_t u0 = 0x0000bf80u, u1 = 0xbf800000u;
+  asm("" : "+v"(u0), "+v"(u1));
+  const v2bf e0 = __builtin_bit_cast(v2bf, u0), e1 = __builtin_bit_cast(v2bf, u1);
+  const float l0 = __builtin_amdgcn_fdot2_f32_bf16(h01, e0, v[0], false);
+  const float l1 = __builtin_amdgcn_fdot2_f32_bf16(h01, e1, v[1], false);
+  const float l2 = __builtin_amdgcn_fdot2_f32_bf16(h23, e0, v[2], false);
+  const float l3 = __builtin_amdgcn_fdot2_f32_bf16(h23, e1, v[3], false);
+  const v2bf l01 = {(__bf16)l0, (__bf16)l1}, l23 = {(__bf16)l2, (__bf16)l3};
   hi = v4bf{h01[0], h01[1], h23[0], h23[1]};
   lo = v4bf{l01[0], l01[1], l23[0], l23[1]};
 }
