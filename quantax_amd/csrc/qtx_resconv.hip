@@ -783,7 +783,8 @@ __device__ __attribute__((always_inline)) inline void rc_forward_rows(const RcGe
 // prologue and pipeline drain are spread over twice the rows, and E_loc at
 // 16 x 16 runs at 290 against 283 TFLOP/s (352 against 334 with ReLU).
 constexpr int RC_WSTAGE = 12288;  // one layer's fragments: bf16x3 6 x 64 x 32 B; FP32 9216 B
-constexpr int rc_win_lds(int B) { return RC_WSTAGE; }  // one weight stage
+constexpr int RC_WBIAS = 64;     // the layer's 16 FP32 biases, after the fragments
+constexpr int rc_win_lds(int B) { return RC_WSTAGE + RC_WBIAS; }  // one weight stage
 typedef __attribute__((address_space(3))) void rc_lds_void;
 
 template <int ACT>
@@ -882,6 +883,11 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
     for (int c = wv; c < nchunk; c += W)
       __builtin_amdgcn_global_load_lds((const void*)(src + c * 1024 + lane * 16), (rc_lds_void*)(dst + c * 1024), 16,
                                        0, 0);
+    // the biases travel with the fragments: read from global memory at the
+    // layer start, the first MFMA would wait for the whole DMA as well
+    if (wv == W - 1 && lane < C / 4)
+      __builtin_amdgcn_global_load_lds((const void*)(packed + bias_off * 4 + (int64_t)l * C * 4 + lane * 16),
+                                       (rc_lds_void*)(dst + RC_WSTAGE), 16, 0, 0);
   };
 
   v4f resr[RPW];  // residual stream of the wave's rows, MFMA output layout
@@ -949,12 +955,11 @@ __device__ __attribute__((always_inline)) inline void rc_forward_win(
         wl[i] = src[WAVE];
       }
     }
+    const v4f bias4 = *reinterpret_cast<const v4f*>(fr + RC_WSTAGE + 16 * gq);
     // every wave holds this layer's fragments: the next layer's travel into
     // the stage while this one runs
     __syncthreads();
     if (l + 1 < 2 * nb) stage_weights(l + 1);
-    const float* bias = pf + bias_off + (int64_t)l * C;
-    const v4f bias4 = v4f{bias[4 * gq], bias[4 * gq + 1], bias[4 * gq + 2], bias[4 * gq + 3]};
     v4f accp = bias4;
 #pragma unroll
     for (int i = 0; i <= RPW; ++i) {
@@ -1417,7 +1422,18 @@ __global__ __launch_bounds__(W * 64, (ROWS == 2 ? (W == 4 * B ? 3 : 2) : (ROWS =
   // with at least two waves per chain, the FP64 acceptance threshold of
   // chain b is computed by wave B + b while wave b draws the move
   constexpr bool kThrWave = (2 * B <= W);
+  static_assert(W >= B, "one wave per chain at most");
+  // the step's Philox words are fetched one step ahead (their HBM latency
+  // hidden by the forward): waves 0 .. B - 1 draw chain wv's move, waves
+  // B .. 2 B - 1 its acceptance threshold
+  const int cbw = (wv < B) ? wv : ((kThrWave && wv < 2 * B) ? wv - B : -1);
+  const int64_t cw = c0 + cbw;
+  const bool wok = cbw >= 0 && cw < ns;
+  uint4 wn = make_uint4(0u, 0u, 0u, 0u);
+  if (wok && nsteps > 0) wn = reinterpret_cast<const uint4*>(words)[cw];
   for (int t = 0; t < nsteps; ++t) {
+    const uint4 wcur = wn;
+    if (wok && t + 1 < nsteps) wn = reinterpret_cast<const uint4*>(words)[(int64_t)(t + 1) * ns + cw];
     // the lane index through an opaque move, as in the forwards: the step's
     // per-lane LDS addresses are formed here, not held across the loop
     int lane = threadIdx.x % WAVE;
@@ -1429,7 +1445,7 @@ __global__ __launch_bounds__(W * 64, (ROWS == 2 ? (W == 4 * B ? 3 : 2) : (ROWS =
       bool upd = false;
       double thr = 0.0;
       if (c < ns) {
-        uint4 w = reinterpret_cast<const uint4*>(words)[(int64_t)t * ns + c];
+        const uint4 w = wcur;
         propose_move(kind, cur + b * N, N, nbr, width, hopping, nhop[b], w.x, w.y, lane, fa, fb, upd);
         if constexpr (!kThrWave) thr = accept_threshold(w.z, w.w);
       }
@@ -1450,10 +1466,7 @@ __global__ __launch_bounds__(W * 64, (ROWS == 2 ? (W == 4 * B ? 3 : 2) : (ROWS =
         const int b = wv - B;
         const int64_t c = c0 + b;
         double thr = 0.0;
-        if (c < ns) {
-          uint4 w = reinterpret_cast<const uint4*>(words)[(int64_t)t * ns + c];
-          thr = accept_threshold(w.z, w.w);
-        }
+        if (c < ns) thr = accept_threshold(wcur.z, wcur.w);
         if (lane == 0) mv_thr[b] = thr;
       }
     }
@@ -1576,30 +1589,33 @@ __global__ __launch_bounds__(W * 64, (ROWS == 2 ? (W == 4 * B ? 3 : 2) : (ROWS =
   }
   __syncthreads();
   for (int start = 0; start < nvalid; start += B) {
+    // thread b < B: the next candidate's sites and this candidate's coupling,
+    // fetched before the forward and held in registers across it (their
+    // global-memory latency hidden by the forward)
+    int na = -1, nb = -1;
+    double cb = 0.0;
     if (threadIdx.x < B) {
       const int b = threadIdx.x;
-      int na = -1, nb = -1;
       if (start + B + b < nvalid) {
         const int k = vlist[start + B + b];
         na = H.off_a[k];
         nb = H.off_b[k];
       }
-      mv[4 * b + 2] = (int16_t)na;
-      mv[4 * b + 3] = (int16_t)nb;
+      if (start + b < nvalid) cb = H.off_c[vlist[start + b]];
     }
     RC_FWD(prop, lp);
-    if (threadIdx.x == 0) {
+    if (wv == 0) {
+      // sum in candidate order by thread 0, as the couplings' owners pass them on
+      double term = 0.0;
+      if (lane < B && start + lane < nvalid) term = cb * exp_ratio(lp[lane] - lp0);
       double e = e_s;
-      for (int b = 0; b < B && start + b < nvalid; ++b) {
-        const double r = exp_ratio(lp[b] - lp0);
-        e += H.off_c[vlist[start + b]] * r;
-      }
-      e_s = e;
+      for (int b = 0; b < B && start + b < nvalid; ++b) e += __shfl(term, b);
+      if (lane == 0) e_s = e;
     }
     if (threadIdx.x < B) {
       const int b = threadIdx.x;
       int8_t* pb = prop + b * N;
-      const int fa = mv[4 * b], fb = mv[4 * b + 1], na = mv[4 * b + 2], nb = mv[4 * b + 3];
+      const int fa = mv[4 * b], fb = mv[4 * b + 1];
       if (fa >= 0) pb[fa] = s0[fa];
       if (fb >= 0) pb[fb] = s0[fb];
       if (na >= 0) pb[na] = -s0[na];
