@@ -59,6 +59,13 @@ extern "C" {
  * nothing.  Without it that scratch is allocated by rocBLAS, once per
  * (thread, device, stream) handle, and kept. */
 #define QTX_SOLVE_BLAS_WS 0x800
+/* flag: EIGH_PINV takes the Chebyshev route whenever the cutoff allows it (c
+ * >= 1e-5 lambda_max), whatever n; without it that route is taken from
+ * n = 2048 on.  The route: one Cholesky factorisation of T + c I, its
+ * triangular inverse P, then the soft inverse as a Chebyshev series in
+ * (T + c I)^-1 T = I - c P^T P (about 158 terms at 1e-15 of the largest
+ * coefficient). */
+#define QTX_SOLVE_CHEB 0x1000
 
 /* ResConv forward precision */
 #define QTX_CONV_FP32 0   /* FP32 products on FP32 MFMA */
@@ -372,14 +379,18 @@ int qtx_pinv_eig(void* stream, const double* T, int64_t n, const double* b, doub
                  double* x, void* ws, size_t ws_bytes);
 
 /* What the calling thread's last qtx_minsr_solve did: path 0 Cholesky shift,
- * 1 partial fractions, 2 dsyevd, 3 shifted Cholesky (small cutoffs) (-1: none
+ * 1 partial fractions, 2 dsyevd, 3 shifted Cholesky (small cutoffs), 4
+ * Chebyshev series (-1: none
  * or failed); Lanczos steps and the lambda_max they gave (0 when not run);
  * fallback (nullable): why EIGH_PINV took dsyevd: 0 it did not, 1 non-finite
  * cutoff or a small cutoff with reweighted MinSR, 2 non-finite Lanczos
- * coefficients (non-finite T), 3 lambda_max <= 0, 4 a partial-fraction
- * Cholesky factorisation failed, 5 SNR truncation (needs the eigenvectors), 6
+ * coefficients (non-finite T), 3 lambda_max <= 0, 4 a partial-fraction or
+ * Chebyshev-route Cholesky factorisation (or triangular inverse) failed, 5 SNR
+ * truncation (needs the eigenvectors), 6
  * the shifted factorisation failed (an eigenvalue of T_d at or below 150 c),
- * 7 its refinement did not converge.  Diagnostic only: no counterpart in the
+ * 7 its refinement did not converge, 8 the Chebyshev series grew (T has
+ * eigenvalues below about -0.048 c, from rounding in the Gram): the partial
+ * fractions ran instead (path 1).  Diagnostic only: no counterpart in the
  * reference, which always runs eigh. */
 int qtx_minsr_last_path(int* path, int* lanczos_steps, double* lambda_max, int* fallback);
 

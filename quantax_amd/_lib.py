@@ -19,6 +19,7 @@ QTX_LOCALFLIP, QTX_EXCHANGE = 0, 1
 QTX_GRAM_FP64, QTX_GRAM_FP32, QTX_GRAM_BF16X3 = 0, 1, 2
 QTX_SOLVE_CHOL_SHIFT, QTX_SOLVE_EIGH_PINV, QTX_SOLVE_EIGH_PINV_DIRECT = 0, 1, 2
 QTX_SOLVE_MINNORM, QTX_SOLVE_LSTSQ, QTX_SOLVE_SNR, QTX_SOLVE_BLAS_WS = 0x100, 0x200, 0x400, 0x800
+QTX_SOLVE_CHEB = 0x1000
 QTX_CONV_FP32, QTX_CONV_BF16X3 = 0, 1
 QTX_ACT_GELU_TANH, QTX_ACT_GELU_ERF, QTX_ACT_RELU = 0, 1, 2
 TAG_SWEEP, TAG_INIT = 0x51A7, 0x1D17

@@ -847,6 +847,107 @@ __global__ void refine_accum_kernel(const double* __restrict__ Z, int64_t n, int
   x[i] = first ? v : x[i] + v;
 }
 
+// ---------------------------------------------------------------- soft inverse by a Chebyshev expansion
+// With G = T + sigma I = L L^T and P = L^-1 (rocSOLVER's lower triangle, i.e.
+// row r of the row-major n x n array holds P_{c r} at columns c >= r), M =
+// T G^-1 = I - sigma P^T P is symmetric with the eigenvectors of T and
+// eigenvalues mu = v / (v + sigma) in [0, mu_max]; g(T) = h(M) with h(mu) =
+// g(sigma mu / (1 - mu)), a function analytic on [0, mu_max] whose nearest
+// poles (from those of g at v = c e^{i pi (2k+1)/6}) stay ~0.13 away for
+// sigma = c, so its Chebyshev series converges by ~1.3 per term.  Each term
+// costs one P product and one P^T product, both half-row streams of the
+// array once the strictly lower part (row-major) holds the transpose of the
+// upper one.
+
+// S[r n + c] = S[c n + r] for c < r: 64 x 64 tiles through LDS (rows padded
+// to 65 doubles against bank conflicts); one workgroup per lower tile
+__global__ __launch_bounds__(256) void tri_mirror_kernel(double* __restrict__ S, int64_t n, int nt) {
+  __shared__ double buf[64][65];
+  // lower tile (tr, tc), tr >= tc, from the linear index
+  int64_t q = blockIdx.x;
+  int tr = (int)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
+  while ((int64_t)(tr + 1) * (tr + 2) / 2 <= q) ++tr;
+  while ((int64_t)tr * (tr + 1) / 2 > q) --tr;
+  const int tc = (int)(q - (int64_t)tr * (tr + 1) / 2);
+  if (tr >= nt) return;
+  const int64_t r0 = (int64_t)tr * 64, c0 = (int64_t)tc * 64;
+  const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+  // source tile: rows c0.., columns r0.. (the upper part)
+  for (int a = ty; a < 64; a += 4) {
+    const int64_t sr = c0 + a, sc = r0 + tx;
+    buf[a][tx] = (sr < n && sc < n) ? S[sr * n + sc] : 0.0;
+  }
+  __syncthreads();
+  for (int b = ty; b < 64; b += 4) {
+    const int64_t r = r0 + b, c = c0 + tx;
+    if (r < n && c < n && c < r) S[r * n + c] = buf[tx][b];
+  }
+}
+
+// Y = P X (LOWER: row i, columns c <= i) or P^T X (upper: columns c >= i):
+// one wave per row, coalesced reads along the row
+template <bool LOWER>
+__global__ __launch_bounds__(256) void tri_rows_kernel(const double* __restrict__ S, int64_t n,
+                                                       const double* __restrict__ X, double* __restrict__ Y) {
+  const int lane = threadIdx.x % WAVE;
+  const int64_t i = (int64_t)blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE;
+  if (i >= n) return;
+  const double* row = S + i * n;
+  const int64_t lo = LOWER ? 0 : i, hi = LOWER ? i + 1 : n;
+  double acc = 0.0;
+  int64_t j = lo + lane;
+  for (; j + 3 * WAVE < hi; j += 4 * WAVE)
+    acc += row[j] * X[j] + row[j + WAVE] * X[j + WAVE] + row[j + 2 * WAVE] * X[j + 2 * WAVE] +
+           row[j + 3 * WAVE] * X[j + 3 * WAVE];
+  for (; j < hi; j += WAVE) acc += row[j] * X[j];
+  acc = wave_sum(acc);
+  if (lane == 0) Y[i] = acc;
+}
+
+// One Chebyshev term: w = P^T u (upper half rows) fused with the recurrence
+// t_{k+1} = 2 (alpha t_k - beta w) - t_{k-1} (first: t_1 = alpha t_0 - beta w,
+// x = a0 t_0 + a t_1), x += a t_{k+1}; A = alpha I - beta P^T P maps the
+// spectrum of M onto [-1, 1].  t_{k+1} replaces t_{k-1} in tprev.
+__global__ __launch_bounds__(256) void cheb_rows_kernel(const double* __restrict__ S, int64_t n,
+                                                        const double* __restrict__ u, const double* __restrict__ tcur,
+                                                        double* __restrict__ tprev, double alpha, double beta,
+                                                        double a0, double a, int first, double* __restrict__ x) {
+  const int lane = threadIdx.x % WAVE;
+  const int64_t i = (int64_t)blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE;
+  if (i >= n) return;
+  const double* row = S + i * n;
+  double acc = 0.0;
+  int64_t j = i + lane;
+  for (; j + 3 * WAVE < n; j += 4 * WAVE)
+    acc += row[j] * u[j] + row[j + WAVE] * u[j + WAVE] + row[j + 2 * WAVE] * u[j + 2 * WAVE] +
+           row[j + 3 * WAVE] * u[j + 3 * WAVE];
+  for (; j < n; j += WAVE) acc += row[j] * u[j];
+  const double w = wave_sum(acc);
+  if (lane == 0) {
+    const double tk = tcur[i];
+    const double tn = first ? alpha * tk - beta * w : 2.0 * (alpha * tk - beta * w) - tprev[i];
+    tprev[i] = tn;
+    x[i] = first ? a0 * tk + a * tn : x[i] + a * tn;
+  }
+}
+
+// single block: out[0] = |a|^2, out[1] = |b|^2
+__global__ __launch_bounds__(1024) void sq_norms_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                                                        int64_t n, double* __restrict__ out) {
+  __shared__ double red[16];
+  double sa = 0.0, sb = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    sa += a[i] * a[i];
+    sb += b[i] * b[i];
+  }
+  sa = block_sum_1024(sa, red);
+  sb = block_sum_1024(sb, red);
+  if (threadIdx.x == 0) {
+    out[0] = sa;
+    out[1] = sb;
+  }
+}
+
 // ---------------------------------------------------------------- SNR truncation
 // _sum_without_noise (optimizer/solver.py:114-125): rho_s = sum_t c_ts is
 // divided by 1 + (tol / snr_s)^6, snr_s = |mean_s| / sqrt(var_s / m), with the
@@ -985,6 +1086,24 @@ static inline bool use_minnorm(int64_t ns, int64_t np, int mode) {
   return ns < np;
 }
 constexpr double kSnrMin = 1e-6;  // tol_snr at or below it: no truncation (solver.py:124)
+// Chebyshev route (cutoffs c >= 1e-5 lambda_max): taken from n >= kChebMinN
+// on, where its one factorisation and triangular inverse plus ~2 n^2 x 8 B
+// of streaming per term cost less than the partial fractions' T^2 and three
+// factorisations (measured: 6.4 against 7.1 ms at n = 2048, 15 against 18 ms
+// at 4096, 2.4 against 2.0 ms at 512), or whenever QTX_SOLVE_CHEB asks for
+// it.  Terms up to kChebMax; the series is cut where 8 consecutive
+// coefficients fall below kChebTol of the largest (about 158 terms).
+constexpr int64_t kChebMinN = 2048;
+constexpr int kChebMax = 400;
+constexpr int kChebNodes = 1024;
+constexpr double kChebTol = 1e-15;
+// The series covers mu down to -kChebMargin (eigenvalues of T down to -c / 6,
+// which rounding in the split-bf16 Gram can produce), and the last term's norm
+// is checked against the right-hand side's: a Chebyshev polynomial is bounded
+// by 1 on the interval, so growth means T has eigenvalues below it, and the
+// solve takes the partial fractions instead.
+constexpr double kChebMargin = 0.2;
+constexpr double kChebGrowth = 1.5;
 // shifted Cholesky route: G = T_d - kShiftCut c I must factor, which puts every
 // eigenvalue of T_d above 150 c, where the soft inverse differs from 1/v by
 // (c/v)^6 <= 8.8e-14; kShiftIters preconditioned CG steps, accepted when the
@@ -1455,6 +1574,71 @@ static int blocked_potrf(rocblas_handle h, hipStream_t st, int64_t n, double* A,
 // delayed workgroups — common when other tasks occupy the GPU — then factored
 // garbage (non-positive pivots in 249 of 640 overlapped solves); the tile is
 // now staged and stored by the next launch.
+// Chebyshev coefficients of h(mu) = g(sigma mu / (1 - mu)), g(v) = v^5 /
+// (v^6 + c^6), on [mu_lo, mu_max] (mu = mu_lo + (mu_max - mu_lo) (u + 1) / 2,
+// u in [-1, 1]) from kChebNodes Chebyshev points (a discrete cosine
+// transform); returns the number of terms kept, 0 when the series does not
+// settle within kChebMax.
+static int cheb_coeffs_raw(double c, double sigma, double mu_lo, double mu_max, std::vector<double>& a) {
+  const int N = kChebNodes;
+  std::vector<double> f(N);
+  const double pi = 3.14159265358979323846;
+  for (int j = 0; j < N; ++j) {
+    const double u = cos(pi * (j + 0.5) / N), mu = mu_lo + (mu_max - mu_lo) * (u + 1.0) * 0.5;
+    const double tt = sigma * mu / ((1.0 - mu) * c);  // v / c, negative below mu = 0
+    const double t5 = tt * tt * tt * tt * tt;
+    f[j] = t5 / (t5 * tt + 1.0) / c;
+  }
+  const int K = kChebMax + 8;
+  a.assign(K, 0.0);
+  for (int j = 0; j < N; ++j) {  // cos(k theta_j) by the three-term recurrence
+    const double c1 = cos(pi * (j + 0.5) / N);
+    double ckm = 1.0, ck = c1;
+    a[0] += f[j];
+    a[1] += f[j] * c1;
+    for (int k = 2; k < K; ++k) {
+      const double cn = 2.0 * c1 * ck - ckm;
+      a[k] += f[j] * cn;
+      ckm = ck;
+      ck = cn;
+    }
+  }
+  double amax = 0.0;
+  for (int k = 0; k < K; ++k) {
+    a[k] *= (k == 0 ? 1.0 : 2.0) / N;
+    amax = std::max(amax, fabs(a[k]));
+  }
+  for (int k = 8; k < kChebMax; ++k) {
+    bool small = true;
+    for (int m = k; m < k + 8; ++m) small = small && fabs(a[m]) < kChebTol * amax;
+    if (small) {
+      a.resize(k);
+      return k;
+    }
+  }
+  return 0;
+}
+
+// sigma = c: the series of c h depends on mu_max = 1 / (1 + rel) alone (rel =
+// c / lambda_max), so it is computed once per rel and thread
+static int cheb_coeffs(double c, double rel, std::vector<double>& a) {
+  static thread_local std::vector<std::pair<double, std::vector<double>>> cache;
+  for (const auto& e : cache)
+    if (e.first == rel) {
+      a = e.second;
+      for (double& v : a) v /= c;
+      return (int)a.size();
+    }
+  std::vector<double> u;
+  const int k = cheb_coeffs_raw(1.0, 1.0, -kChebMargin, 1.0 / (1.0 + rel), u);
+  if (k == 0) return 0;
+  if (cache.size() >= 16) cache.erase(cache.begin());
+  cache.emplace_back(rel, u);
+  a = u;
+  for (double& v : a) v /= c;
+  return k;
+}
+
 struct Task {
   qtx_minsr_task* io = nullptr;
   hipStream_t st = nullptr;
@@ -1467,6 +1651,9 @@ struct Task {
   bool need_syevd = false;  // phase 2 runs the dsyevd route
   bool pf = false;          // partial-fraction factorisations enqueued
   bool cf = false;          // shifted Cholesky route enqueued (small cutoffs)
+  bool cheb = false;        // Chebyshev route enqueued
+  int cheb_terms = 0;
+  double c = 0.0, rel = 0.0;  // the cutoff and c / lambda_max
   int mcap = 0, done = 0;   // Lanczos steps available / enqueued
   double lmax = 0.0;
   int lanczos = 0;
@@ -1562,6 +1749,76 @@ static int snr_syevd(Task& t, rocblas_handle h) {
   qtx_minsr_task& io = *t.io;
   SnrRows rows{io.o_dtype, io.O, io.mean, io.r, io.ebar, io.ns, io.np};
   return pinv_syevd(t.st, h, t.n, io.rtol, io.atol, t.w, io.tol_snr, t.minnorm ? nullptr : &rows);
+}
+
+// Partial-fraction route: T^2 by dsyrk, the three quadratics (T - a_k)^2 + b_k^2
+// factored together, and iterative refinement of the complex shifted systems
+// against T itself.
+static int pf_enqueue(Task& t, rocblas_handle h, double c, double rel) {
+  // refinement contracts by about 4 u / rel^2 per pass (4.4e-8 at rel = 1e-4,
+  // 4.4e-6 at 1e-5): two passes reach ~1e-15 for rel >= 1e-4, three for 1e-5
+  const int iters = rel >= 1e-4 ? 2 : (rel >= 1e-5 ? 3 : 4);
+  const rocblas_int ni = (rocblas_int)t.n;
+  const double one = 1.0, zero = 0.0;
+  const unsigned gb = (unsigned)nblk(t.n, 256);
+  const unsigned gm = (unsigned)((t.n + 3) / 4);
+  int rc = rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, ni, ni, &one, t.w.T, ni, &zero,
+                            t.w.T2, ni),
+              "rocblas_dsyrk");
+  if (rc) return rc;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(nblk(3 * kInfoPerFactor, 256)), dim3(256), 0, t.st, t.w.info,
+                     (int64_t)(3 * kInfoPerFactor), 0);
+  // pole k = c e^{i pi (2k+1)/6}: quadratic k into matrix k of the T2 block,
+  // k = 0 last because it overwrites T^2 in place
+  const int64_t nn = t.n * t.n;
+  double pa[3], pb[3];
+  for (int k = 0; k < 3; ++k) {
+    const double th = 3.14159265358979323846 * (2 * k + 1) / 6.0;
+    pa[k] = (k == 1) ? 0.0 : c * cos(th);
+    pb[k] = c * sin(th);
+  }
+  const unsigned gq = (unsigned)std::min<int64_t>((nn + 255) / 256, kGridCap);
+  for (int k : {1, 2, 0}) {
+    hipLaunchKernelGGL(form_quad_kernel, dim3(gq), dim3(256), 0, t.st, t.w.T2, t.w.T, t.n, 2.0 * pa[k], c * c,
+                       t.w.T2 + k * nn);
+    QTX_LAUNCH_CHECK();
+  }
+  rc = blocked_potrf(h, t.st, t.n, t.w.T2, nn, 3, t.w.info, kInfoPerFactor, t.w.dtile);
+  if (rc) return rc;
+  // iterative refinement of the three complex systems (T - p_k) z = y,
+  // the inner solves batched over the three factors, the products with T
+  // over all six vectors in one pass
+  for (int k = 0; k < 3; ++k)
+    hipLaunchKernelGGL(refine_init_kernel, dim3(gb), dim3(256), 0, t.st, t.w.y, t.n, t.w.R + 2 * k * t.n,
+                       t.w.Z + 2 * k * t.n);
+  for (int it = 0; it < iters; ++it) {
+    rc = rb(rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                                          rocblas_diagonal_non_unit, ni, 2, &one, t.w.T2, ni, nn, t.w.R, ni,
+                                          2 * t.n, 3),
+            "rocblas_dtrsm_strided_batched");
+    if (rc) return rc;
+    rc = rb(rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
+                                          rocblas_diagonal_non_unit, ni, 2, &one, t.w.T2, ni, nn, t.w.R, ni,
+                                          2 * t.n, 3),
+            "rocblas_dtrsm_strided_batched");
+    if (rc) return rc;
+    hipLaunchKernelGGL(matvec_rows_kernel<6>, dim3(gm), dim3(256), 0, t.st, t.w.T, t.n, t.w.R, t.w.TW);
+    for (int k = 0; k < 3; ++k)
+      hipLaunchKernelGGL(refine_update_kernel, dim3(gb), dim3(256), 0, t.st, t.w.R + 2 * k * t.n,
+                         t.w.TW + 2 * k * t.n, t.n, pa[k], pb[k], t.w.Z + 2 * k * t.n);
+    if (it + 1 < iters) {
+      hipLaunchKernelGGL(matvec_rows_kernel<6>, dim3(gm), dim3(256), 0, t.st, t.w.T, t.n, t.w.Z, t.w.TW);
+      for (int k = 0; k < 3; ++k)
+        hipLaunchKernelGGL(refine_residual_kernel, dim3(gb), dim3(256), 0, t.st, t.w.y, t.w.Z + 2 * k * t.n,
+                           t.w.TW + 2 * k * t.n, t.n, pa[k], pb[k], t.w.R + 2 * k * t.n);
+    }
+  }
+  for (int k = 0; k < 3; ++k)
+    hipLaunchKernelGGL(refine_accum_kernel, dim3(gb), dim3(256), 0, t.st, t.w.Z + 2 * k * t.n, t.n, k == 0 ? 1 : 0,
+                       t.w.x);
+  QTX_LAUNCH_CHECK();
+  t.pf = true;
+  return QTX_OK;
 }
 
 static int phase2(Task& t, rocblas_handle h) {
@@ -1662,70 +1919,56 @@ static int phase2(Task& t, rocblas_handle h) {
                          t.w.scalars + 6);
       QTX_LAUNCH_CHECK();
       t.cf = true;
-    } else {
-      // refinement contracts by about 4 u / rel^2 per pass (4.4e-8 at rel = 1e-4,
-      // 4.4e-6 at 1e-5): two passes reach ~1e-15 for rel >= 1e-4, three for 1e-5
-      const int iters = rel >= 1e-4 ? 2 : (rel >= 1e-5 ? 3 : 4);
+    }
+    t.c = c;
+    t.rel = rel;
+    std::vector<double> ca;
+    // sigma = c; mu_max from rel slightly reduced: the Ritz value may sit
+    // just below lambda_max
+    const double sigma = c;
+    const double rel_s = rel / (1.0 + 1e-9);
+    const double mu_max = 1.0 / (1.0 + rel_s), mu_lo = -kChebMargin;
+    if (!t.need_syevd && !t.cf && (t.n >= kChebMinN || (io.mode & QTX_SOLVE_CHEB)) &&
+        (t.cheb_terms = cheb_coeffs(c, rel_s, ca)) > 0) {
+      // Chebyshev route: G = T + sigma I = L L^T, P = L^-1, then
+      // x = sum_k a_k T_k(A) y with A = (2 / mu_max) (I - sigma P^T P) - I
       const rocblas_int ni = (rocblas_int)t.n;
-      const double one = 1.0, zero = 0.0;
-      const unsigned gb = (unsigned)nblk(t.n, 256);
+      const int64_t nn = t.n * t.n;
+      const unsigned gq = (unsigned)std::min<int64_t>((nn + 255) / 256, kGridCap);
       const unsigned gm = (unsigned)((t.n + 3) / 4);
-      int rc = rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, ni, ni, &one, t.w.T, ni, &zero,
-                                t.w.T2, ni),
-                  "rocblas_dsyrk");
-      if (rc) return rc;
       hipLaunchKernelGGL(fill_i32_kernel, dim3(nblk(3 * kInfoPerFactor, 256)), dim3(256), 0, t.st, t.w.info,
                          (int64_t)(3 * kInfoPerFactor), 0);
-      // pole k = c e^{i pi (2k+1)/6}: quadratic k into matrix k of the T2 block,
-      // k = 0 last because it overwrites T^2 in place
-      const int64_t nn = t.n * t.n;
-      double pa[3], pb[3];
-      for (int k = 0; k < 3; ++k) {
-        const double th = 3.14159265358979323846 * (2 * k + 1) / 6.0;
-        pa[k] = (k == 1) ? 0.0 : c * cos(th);
-        pb[k] = c * sin(th);
-      }
-      const unsigned gq = (unsigned)std::min<int64_t>((nn + 255) / 256, kGridCap);
-      for (int k : {1, 2, 0}) {
-        hipLaunchKernelGGL(form_quad_kernel, dim3(gq), dim3(256), 0, t.st, t.w.T2, t.w.T, t.n, 2.0 * pa[k], c * c,
-                           t.w.T2 + k * nn);
-        QTX_LAUNCH_CHECK();
-      }
-      rc = blocked_potrf(h, t.st, t.n, t.w.T2, nn, 3, t.w.info, kInfoPerFactor, t.w.dtile);
-      if (rc) return rc;
-      // iterative refinement of the three complex systems (T - p_k) z = y,
-      // the inner solves batched over the three factors, the products with T
-      // over all six vectors in one pass
-      for (int k = 0; k < 3; ++k)
-        hipLaunchKernelGGL(refine_init_kernel, dim3(gb), dim3(256), 0, t.st, t.w.y, t.n, t.w.R + 2 * k * t.n,
-                           t.w.Z + 2 * k * t.n);
-      for (int it = 0; it < iters; ++it) {
-        rc = rb(rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                                              rocblas_diagonal_non_unit, ni, 2, &one, t.w.T2, ni, nn, t.w.R, ni,
-                                              2 * t.n, 3),
-                "rocblas_dtrsm_strided_batched");
-        if (rc) return rc;
-        rc = rb(rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
-                                              rocblas_diagonal_non_unit, ni, 2, &one, t.w.T2, ni, nn, t.w.R, ni,
-                                              2 * t.n, 3),
-                "rocblas_dtrsm_strided_batched");
-        if (rc) return rc;
-        hipLaunchKernelGGL(matvec_rows_kernel<6>, dim3(gm), dim3(256), 0, t.st, t.w.T, t.n, t.w.R, t.w.TW);
-        for (int k = 0; k < 3; ++k)
-          hipLaunchKernelGGL(refine_update_kernel, dim3(gb), dim3(256), 0, t.st, t.w.R + 2 * k * t.n,
-                             t.w.TW + 2 * k * t.n, t.n, pa[k], pb[k], t.w.Z + 2 * k * t.n);
-        if (it + 1 < iters) {
-          hipLaunchKernelGGL(matvec_rows_kernel<6>, dim3(gm), dim3(256), 0, t.st, t.w.T, t.n, t.w.Z, t.w.TW);
-          for (int k = 0; k < 3; ++k)
-            hipLaunchKernelGGL(refine_residual_kernel, dim3(gb), dim3(256), 0, t.st, t.w.y, t.w.Z + 2 * k * t.n,
-                               t.w.TW + 2 * k * t.n, t.n, pa[k], pb[k], t.w.R + 2 * k * t.n);
-        }
-      }
-      for (int k = 0; k < 3; ++k)
-        hipLaunchKernelGGL(refine_accum_kernel, dim3(gb), dim3(256), 0, t.st, t.w.Z + 2 * k * t.n, t.n, k == 0 ? 1 : 0,
-                           t.w.x);
+      hipLaunchKernelGGL(form_shifted_kernel, dim3(gq), dim3(256), 0, t.st, t.w.T, t.n, 0.0, -sigma, t.w.T2);
       QTX_LAUNCH_CHECK();
-      t.pf = true;
+      int rc = blocked_potrf(h, t.st, t.n, t.w.T2, nn, 1, t.w.info, kInfoPerFactor, t.w.dtile);
+      if (rc) return rc;
+      if ((rc = rb(rocsolver_dtrtri(h, rocblas_fill_lower, rocblas_diagonal_non_unit, ni, t.w.T2, ni,
+                                    t.w.info + kInfoPerFactor),
+                   "rocsolver_dtrtri")))
+        return rc;
+      const int nt = (int)((t.n + 63) / 64);
+      hipLaunchKernelGGL(tri_mirror_kernel, dim3((unsigned)((int64_t)nt * (nt + 1) / 2)), dim3(256), 0, t.st, t.w.T2,
+                         t.n, nt);
+      double* B[2] = {t.w.R, t.w.R + t.n};  // t_k, t_{k +- 1}
+      double* u = t.w.R + 2 * t.n;
+      hipLaunchKernelGGL(copy_f64_kernel, dim3(nblk(t.n, 256)), dim3(256), 0, t.st, t.w.y, B[0], t.n);
+      const double span = mu_max - mu_lo;
+      const double alpha = (2.0 - mu_max - mu_lo) / span, beta = 2.0 * sigma / span;
+      for (int k = 0; k + 1 < t.cheb_terms; ++k) {
+        const double* tcur = B[k & 1];
+        double* tprev = B[(k + 1) & 1];
+        hipLaunchKernelGGL(tri_rows_kernel<true>, dim3(gm), dim3(256), 0, t.st, t.w.T2, t.n, tcur, u);
+        hipLaunchKernelGGL(cheb_rows_kernel, dim3(gm), dim3(256), 0, t.st, t.w.T2, t.n, u, tcur, tprev, alpha, beta,
+                           ca[0], ca[k + 1], k == 0 ? 1 : 0, t.w.x);
+      }
+      // |t_last|^2 and |y|^2 for the growth check
+      hipLaunchKernelGGL(sq_norms_kernel, dim3(1), dim3(1024), 0, t.st, B[(t.cheb_terms - 1) & 1], t.w.y, t.n,
+                         t.w.scalars + 4);
+      QTX_LAUNCH_CHECK();
+      t.cheb = true;
+    } else if (!t.need_syevd && !t.cf) {
+      int rc = pf_enqueue(t, h, c, rel);
+      if (rc) return rc;
     }
   }
   if (t.need_syevd) {
@@ -1740,6 +1983,30 @@ template <typename TI>
 static int phase3(Task& t, rocblas_handle h) {
   qtx_minsr_task& io = *t.io;
   const bool chol = base_mode(io.mode) == QTX_SOLVE_CHOL_SHIFT;
+  if (t.cheb) {
+    std::vector<int32_t> info_h(kInfoPerFactor + 1);  // the factorisation's blocks, then dtrtri
+    double nrm[2];                                     // |t_last|^2, |y|^2
+    QTX_HIP(hipMemcpyAsync(info_h.data(), t.w.info, info_h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, t.st));
+    QTX_HIP(hipMemcpyAsync(nrm, t.w.scalars + 4, sizeof(nrm), hipMemcpyDeviceToHost, t.st));
+    QTX_HIP(hipStreamSynchronize(t.st));
+    bool ok = true;
+    for (int32_t v : info_h) ok = ok && (v == 0);
+    const bool bounded = std::isfinite(nrm[0]) && nrm[0] <= kChebGrowth * kChebGrowth * nrm[1];
+    if (ok && bounded) {
+      hipLaunchKernelGGL(copy_f64_kernel, dim3(nblk(t.n, 256)), dim3(256), 0, t.st, t.w.x, t.w.y, t.n);
+      QTX_LAUNCH_CHECK();
+      t.path = 4;
+    } else if (ok) {  // T has eigenvalues below the series' interval: partial fractions
+      t.fallback = 8;
+      int rc = pf_enqueue(t, h, t.c, t.rel);
+      if (rc) return rc;
+    } else {  // T and y are intact: take the dsyevd route
+      t.fallback = 4;
+      int rc = snr_syevd(t, h);
+      if (rc < 0) return rc;
+      t.path = 2;
+    }
+  }
   if (chol || t.pf) {
     std::vector<int32_t> info_h(3 * kInfoPerFactor);
     QTX_HIP(hipMemcpyAsync(info_h.data(), t.w.info, info_h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, t.st));
@@ -1802,7 +2069,8 @@ static int check_task(const qtx_minsr_task& io) {
   const int mode = base_mode(io.mode);
   QTX_REQUIRE(mode == QTX_SOLVE_CHOL_SHIFT || mode == QTX_SOLVE_EIGH_PINV || mode == QTX_SOLVE_EIGH_PINV_DIRECT,
               "unknown solver mode");
-  QTX_REQUIRE((io.mode & ~(0xff | QTX_SOLVE_MINNORM | QTX_SOLVE_LSTSQ | QTX_SOLVE_SNR | QTX_SOLVE_BLAS_WS)) == 0 &&
+  QTX_REQUIRE((io.mode & ~(0xff | QTX_SOLVE_MINNORM | QTX_SOLVE_LSTSQ | QTX_SOLVE_SNR | QTX_SOLVE_BLAS_WS |
+                            QTX_SOLVE_CHEB)) == 0 &&
                   !((io.mode & QTX_SOLVE_MINNORM) && (io.mode & QTX_SOLVE_LSTSQ)),
               "bad solver mode flags");
   QTX_REQUIRE(io.gram == QTX_GRAM_FP64 || io.gram == QTX_GRAM_FP32 || io.gram == QTX_GRAM_BF16X3,
@@ -1860,7 +2128,7 @@ static int blas_query(rocblas_handle h, int64_t ns, int64_t np, int mode, SolveW
     good &= ok(rocblas_dgemv(h, rocblas_operation_transpose, ni, ni, &one, w.T, ni, w.y, 1, &zero, w.z, 1));
     good &= ok(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &one, w.T, ni, w.z, 1, &zero, w.y, 1));
   }
-  if (bm == QTX_SOLVE_EIGH_PINV) {  // the small-cutoff route: inverse factor and its products
+  if (bm == QTX_SOLVE_EIGH_PINV) {  // small-cutoff and Chebyshev routes: inverse factor and its products
     good &= ok(rocsolver_dtrtri(h, rocblas_fill_lower, rocblas_diagonal_non_unit, ni, w.T2, ni, w.info));
     for (rocblas_operation op : {rocblas_operation_none, rocblas_operation_transpose})
       good &= ok(rocblas_dtrmv(h, rocblas_fill_lower, op, rocblas_diagonal_non_unit, ni, w.T2, ni, w.R, 1));

@@ -179,13 +179,15 @@ def solve_step(O: torch.Tensor, Omean: torch.Tensor, ebar: torch.Tensor, solver:
 
 def last_solve_info() -> dict:
     """What the last solve on this thread did: ``path`` is "chol_shift",
-    "partial_fractions" or "syevd"; Lanczos steps and lambda_max when run;
-    ``fallback``: why the partial-fraction route was left, if it was."""
+    "partial_fractions", "syevd", "shifted_cholesky" or "chebyshev"; Lanczos
+    steps and lambda_max when run;
+    ``fallback``: why the partial-fraction or Chebyshev route was left, if it was."""
     path, steps, lmax, fb = C.c_int(), C.c_int(), C.c_double(), C.c_int()
     _lib.call("qtx_minsr_last_path", C.byref(path), C.byref(steps), C.byref(lmax), C.byref(fb))
-    names = {0: "chol_shift", 1: "partial_fractions", 2: "syevd", 3: "shifted_cholesky"}
+    names = {0: "chol_shift", 1: "partial_fractions", 2: "syevd", 3: "shifted_cholesky", 4: "chebyshev"}
     reasons = {0: None, 1: "small_cutoff", 2: "nonfinite", 3: "nonpositive_lambda_max", 4: "cholesky_failed",
-               5: "snr_truncation", 6: "shifted_cholesky_failed", 7: "shifted_cholesky_not_converged"}
+               5: "snr_truncation", 6: "shifted_cholesky_failed", 7: "shifted_cholesky_not_converged",
+               8: "chebyshev_interval"}
     return {"path": names.get(path.value, "none"), "lanczos_steps": steps.value, "lambda_max": lmax.value,
             "fallback": reasons.get(fb.value)}
 

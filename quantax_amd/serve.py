@@ -349,9 +349,10 @@ class Server:
         if rc != 0:
             raise _lib.QtxError(f"qtx_minsr_solve_many failed ({rc}): "
                                 f"{_lib.lib().qtx_last_error().decode(errors='replace')}")
-        names = {0: "chol_shift", 1: "partial_fractions", 2: "syevd", 3: "shifted_cholesky"}
+        names = {0: "chol_shift", 1: "partial_fractions", 2: "syevd", 3: "shifted_cholesky", 4: "chebyshev"}
         reasons = {1: "small_cutoff", 2: "nonfinite", 3: "nonpositive_lambda_max", 4: "cholesky_failed",
-                   5: "snr_truncation", 6: "shifted_cholesky_failed", 7: "shifted_cholesky_not_converged"}
+                   5: "snr_truncation", 6: "shifted_cholesky_failed", 7: "shifted_cholesky_not_converged",
+               8: "chebyshev_interval"}
         for j, tk in zip(active, tasks):
             path = names.get(tk.path, "none") + (f":{reasons[tk.fallback]}" if tk.fallback else "")
             with torch.cuda.stream(j.stream):

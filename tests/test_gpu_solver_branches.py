@@ -8,6 +8,8 @@ oracle (needs an MI355X).
   MinSR and SR branches, with and without reweighting, and across the row
   blocks in which the SR branch forms Obar.
 * ``minsr_pinv_eig`` (``solver.py:262-294``) on indefinite and Gram matrices.
+* the soft pseudo-inverse's Chebyshev route, forced on small cases and by
+  size on larger ones.
 
 Tolerance: FP64 solves 1e-9 relative (summation order against LAPACK; the
 matrices here have condition numbers below 1e5 where the soft cutoff does
@@ -96,6 +98,73 @@ def test_forced_branches_match_oracle(shape, odt):
               f"err {err:.2e}")
         assert info["path"] == path, info
         assert err < 1e-9, err
+
+
+@pytest.mark.parametrize("shape", [(120, 300), (300, 120)])
+@pytest.mark.parametrize("weights", [False, True])
+def test_chebyshev_route_matches_oracle(shape, weights):
+    """The Chebyshev route (QTX_SOLVE_CHEB forces it below its size
+    threshold) on both branches and shapes, the rank-deficient Grams
+    included, with and without reweighting, at cutoffs across the route's
+    range (rtol 1e-2 .. 1e-5), against the oracle's eigh form."""
+    ns, np_ = shape
+    O, e, r = _problem(3, ns, np_, np.float64, weights)
+    Od, mean, eb, rd = _device_inputs(O, e, r)
+    A, b = _oracle(O, e, r)
+    for branch in ("minnorm", "lstsq"):
+        for rtol in (1e-2, 1e-4, 1e-5):
+            solver = getattr(qtx.optimizer, f"{branch}_pinv_eig")(rtol=rtol)
+            solver.mode |= _lib.QTX_SOLVE_CHEB
+            ref = getattr(osr, f"{branch}_pinv_eig")(A, b, rtol=rtol)
+            x = qtx.optimizer.solve_step(Od, mean, eb, solver, rd).cpu().numpy()
+            info = qtx.optimizer.last_solve_info()
+            err = _rel(x, ref)
+            print(f"{shape} weights {weights} {branch} rtol {rtol}: {info['path']} err {err:.2e}")
+            assert info["path"] == "chebyshev", info
+            assert err < 1e-9, err
+
+
+def test_chebyshev_route_by_size_several_blocks():
+    """From n = 2048 on EIGH_PINV takes the Chebyshev route by itself: 2500
+    samples (three 1024-wide diagonal blocks of the factorisation, the last
+    partial; a triangular inverse of the same size), against the oracle."""
+    O, e, _ = _problem(4, 2500, 4000, np.float32)
+    O[:, :2000] *= 0.05  # spread the spectrum
+    Od, mean, eb, _ = _device_inputs(O, e, None)
+    A, b = _oracle(O, e, None)
+    for rtol in (1e-4, 1e-2):
+        x = qtx.optimizer.solve_step(Od, mean, eb, qtx.optimizer.auto_pinv_eig(rtol=rtol, gram="fp64"))
+        info = qtx.optimizer.last_solve_info()
+        err = _rel(x.cpu().numpy(), osr.auto_pinv_eig(A, b, rtol=rtol))
+        print(f"2500 x 4000 rtol {rtol}: {info['path']} err {err:.2e}")
+        assert info["path"] == "chebyshev", info
+        assert err < 1e-9, err
+
+
+def test_chebyshev_route_split_bf16_gram_large():
+    """8192 x 17000 with the split-bf16 Gram, whose rounding leaves T with
+    eigenvalues slightly below zero: the series covers them down to -c / 6
+    and otherwise (its last term grows) the solve takes the partial fractions
+    (fallback "chebyshev_interval").  Either way the result equals the
+    dsyevd route's on the same Gram (the dsyevd route is pinned to the
+    oracle in the smaller cases)."""
+    ns, np_ = 8192, 17000
+    g = torch.Generator(device=DEV).manual_seed(5)
+    O = torch.randn(ns, np_, device=DEV, dtype=torch.float32, generator=g)
+    O *= torch.logspace(0, -3, np_, device=DEV, dtype=torch.float32)[None, :]
+    mean = O.mean(0).double()
+    eb = torch.randn(ns, device=DEV, dtype=torch.float64, generator=g)
+    eb -= eb.mean()
+    for rtol in (1e-4, 1e-5):
+        x = qtx.optimizer.solve_step(O, mean, eb, qtx.optimizer.auto_pinv_eig(rtol=rtol, gram="bf16x3"))
+        info = qtx.optimizer.last_solve_info()
+        direct = qtx.optimizer.Solver(_lib.QTX_SOLVE_EIGH_PINV_DIRECT, rtol=rtol, gram="bf16x3")
+        xd = qtx.optimizer.solve_step(O, mean, eb, direct)
+        err = (torch.linalg.norm(x - xd) / torch.linalg.norm(xd)).item()
+        print(f"8192 x 17000 split-bf16 rtol {rtol}: {info['path']} ({info['fallback']}) err vs dsyevd {err:.2e}")
+        assert info["path"] in ("chebyshev", "partial_fractions"), info
+        assert info["path"] == "chebyshev" or info["fallback"] == "chebyshev_interval", info
+        assert err < 1e-10, err
 
 
 def test_forced_minnorm_split_bf16_gram_with_more_samples():
