@@ -39,7 +39,8 @@ def parse():
                    help="proposals per chain per sweep (default 2N; 8 for c4)")
     p.add_argument("--thermal-sweeps", type=int, default=None, help="thermalisation sweeps (default 1; 0 for c4)")
     p.add_argument("--solver", default=None, choices=["eigh", "eigh-direct", "chol"],
-                   help="eigh: soft pseudo-inverse (partial fractions when the cutoff allows); "
+                   help="eigh: soft pseudo-inverse without eigenvectors when the cutoff allows (Chebyshev "
+                        "series from n = 2048, else partial fractions; shifted Cholesky for small cutoffs); "
                         "eigh-direct: the same through dsyevd; chol: diagonal-shift Cholesky "
                         "(default: the configuration's own, chol for c1/c2, eigh for c3/c4)")
     p.add_argument("--rtol", type=float, default=1e-4, help="soft pseudo-inverse cutoff of the eigh solver")
@@ -298,7 +299,7 @@ def reference_default_solve(last, solver, torch, qtx):
            "fallback": info["fallback"], "dsyevd_ms": round(ms_d, 1),
            "rel_diff_vs_dsyevd": (torch.linalg.norm(x - x_d) / torch.linalg.norm(x_d)).item(),
            "note": "not part of the timed step, which uses rtol = "
-           f"{solver.rtol:g} (partial-fraction route)"}
+           f"{solver.rtol:g} (its route in solve_path)"}
     print(f"[bench] reference-default solve (rtol 1e-12): {rep['ms']} ms via {rep['path']}; dsyevd "
           f"{rep['dsyevd_ms']} ms, relative difference {rep['rel_diff_vs_dsyevd']:.2e}", file=sys.stderr, flush=True)
     return rep
