@@ -1738,10 +1738,11 @@ static int phase1(Task& t, rocblas_handle h) {
   // recurrence; the largest Ritz value is checked every 32 steps)
   hipLaunchKernelGGL(lanczos_init_kernel, dim3(1), dim3(1024), 0, t.st, t.n, t.w.V, t.w.ab + kLanczosMax);
   t.mcap = (int)(t.n < kLanczosMax ? t.n : kLanczosMax);
-  // first convergence check after 32 steps when one product streams more
-  // than 512 MB of T (n > 8192: 16 more products then cost far more than the
-  // host round trip of a check), after 48 otherwise
-  const int first = t.n > 8192 ? 32 : 48;
+  // convergence checks every 16 steps when one product streams more than
+  // 512 MB of T (n > 8192: 16 products cost far more than the host round
+  // trip of a check; a VMC Gram's top eigenvalue, well separated, settles
+  // within 8 steps), else after 48 and then every 32
+  const int first = t.n > 8192 ? 16 : 48;
   return lanczos_enqueue(t, h, t.mcap < first ? t.mcap : first);
 }
 
@@ -1852,7 +1853,8 @@ static int phase2(Task& t, rocblas_handle h) {
       if (exact || t.done >= t.mcap) break;
       double prev = tridiag_max_eig(ha.data(), hb.data(), m - 8);
       if (fabs(lmax - prev) <= 1e-14 * fabs(lmax)) break;
-      int rc = lanczos_enqueue(t, h, t.done + 32 < t.mcap ? t.done + 32 : t.mcap);
+      const int more = t.n > 8192 ? 16 : 32;
+      int rc = lanczos_enqueue(t, h, t.done + more < t.mcap ? t.done + more : t.mcap);
       if (rc) return rc;
     }
     t.lanczos = msteps;
