@@ -586,8 +586,44 @@ __global__ __launch_bounds__(256) void apply_update_kernel(T* __restrict__ theta
 // quadratic factored once by Cholesky; the residual uses T itself, so the
 // result has the accuracy of the well-conditioned complex system.
 
+// One wave's partial sums of row[j] * x[j] over j in [lo, hi): 16-byte loads
+// (1 KB per wave instruction) where row and x share their 16-byte alignment
+// phase, after peeling one element to reach it; 8-byte loads otherwise.
+__device__ __forceinline__ double wave_row_dot(const double* __restrict__ row, const double* __restrict__ x,
+                                               int64_t lo, int64_t hi, int lane) {
+  double acc = 0.0;
+  if (((reinterpret_cast<uintptr_t>(row) ^ reinterpret_cast<uintptr_t>(x)) & 15) == 0) {
+    if ((reinterpret_cast<uintptr_t>(row + lo) & 15) && lo < hi) {
+      if (lane == 0) acc = row[lo] * x[lo];
+      ++lo;
+    }
+    const double2* r2 = reinterpret_cast<const double2*>(row + lo);
+    const double2* x2 = reinterpret_cast<const double2*>(x + lo);
+    const int64_t m = (hi - lo) / 2;
+    int64_t k = lane;
+    for (; k + 3 * WAVE < m; k += 4 * WAVE) {
+      const double2 a0 = r2[k], a1 = r2[k + WAVE], a2 = r2[k + 2 * WAVE], a3 = r2[k + 3 * WAVE];
+      const double2 b0 = x2[k], b1 = x2[k + WAVE], b2 = x2[k + 2 * WAVE], b3 = x2[k + 3 * WAVE];
+      acc += a0.x * b0.x + a0.y * b0.y + a1.x * b1.x + a1.y * b1.y + a2.x * b2.x + a2.y * b2.y + a3.x * b3.x +
+             a3.y * b3.y;
+    }
+    for (; k < m; k += WAVE) {
+      const double2 a0 = r2[k], b0 = x2[k];
+      acc += a0.x * b0.x + a0.y * b0.y;
+    }
+    if (((hi - lo) & 1) && lane == 0) acc += row[hi - 1] * x[hi - 1];
+    return acc;
+  }
+  int64_t j = lo + lane;
+  for (; j + 3 * WAVE < hi; j += 4 * WAVE)
+    acc += row[j] * x[j] + row[j + WAVE] * x[j + WAVE] + row[j + 2 * WAVE] * x[j + 2 * WAVE] +
+           row[j + 3 * WAVE] * x[j + 3 * WAVE];
+  for (; j < hi; j += WAVE) acc += row[j] * x[j];
+  return acc;
+}
+
 // Y = T X for NV vectors (n x NV, column-major) with T symmetric n x n stored
-// in full: one wave per row, coalesced 8-byte reads of the row, X from cache.
+// in full: one wave per row, coalesced reads of the row, X from cache.
 // An HBM stream of T at close to copy bandwidth (library GEMV / GEMM with
 // one or two columns reach a fraction of it).
 template <int NV>
@@ -597,6 +633,11 @@ __global__ __launch_bounds__(256) void matvec_rows_kernel(const double* __restri
   const int64_t i = (int64_t)blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE;
   if (i >= n) return;
   const double* row = T + i * n;
+  if constexpr (NV == 1) {
+    const double s = wave_sum(wave_row_dot(row, X, 0, n, lane));
+    if (lane == 0) Y[i] = s;
+    return;
+  }
   double acc[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = 0.0;
@@ -893,14 +934,7 @@ __global__ __launch_bounds__(256) void tri_rows_kernel(const double* __restrict_
   const int64_t i = (int64_t)blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE;
   if (i >= n) return;
   const double* row = S + i * n;
-  const int64_t lo = LOWER ? 0 : i, hi = LOWER ? i + 1 : n;
-  double acc = 0.0;
-  int64_t j = lo + lane;
-  for (; j + 3 * WAVE < hi; j += 4 * WAVE)
-    acc += row[j] * X[j] + row[j + WAVE] * X[j + WAVE] + row[j + 2 * WAVE] * X[j + 2 * WAVE] +
-           row[j + 3 * WAVE] * X[j + 3 * WAVE];
-  for (; j < hi; j += WAVE) acc += row[j] * X[j];
-  acc = wave_sum(acc);
+  const double acc = wave_sum(wave_row_dot(row, X, LOWER ? 0 : i, LOWER ? i + 1 : n, lane));
   if (lane == 0) Y[i] = acc;
 }
 
@@ -915,14 +949,7 @@ __global__ __launch_bounds__(256) void cheb_rows_kernel(const double* __restrict
   const int lane = threadIdx.x % WAVE;
   const int64_t i = (int64_t)blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE;
   if (i >= n) return;
-  const double* row = S + i * n;
-  double acc = 0.0;
-  int64_t j = i + lane;
-  for (; j + 3 * WAVE < n; j += 4 * WAVE)
-    acc += row[j] * u[j] + row[j + WAVE] * u[j + WAVE] + row[j + 2 * WAVE] * u[j + 2 * WAVE] +
-           row[j + 3 * WAVE] * u[j + 3 * WAVE];
-  for (; j < n; j += WAVE) acc += row[j] * u[j];
-  const double w = wave_sum(acc);
+  const double w = wave_sum(wave_row_dot(S + i * n, u, i, n, lane));
   if (lane == 0) {
     const double tk = tcur[i];
     const double tn = first ? alpha * tk - beta * w : 2.0 * (alpha * tk - beta * w) - tprev[i];
