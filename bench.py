@@ -66,6 +66,9 @@ def parse():
                         "otherwise the rest is extrapolated at the job's later-step cost)")
     p.add_argument("--activation", default="gelu", choices=["gelu", "relu"],
                    help="ResConv activation: gelu (the reference's tanh-form GELU) or the opt-in ReLU variant")
+    p.add_argument("--no-relu-variant", action="store_true",
+                   help="c3 with GELU in both modes: skip the same step with the opt-in ReLU activation, "
+                        "reported beside the headline as relu_variant")
     p.add_argument("--calib-copy-mb", type=int, default=0,
                    help="run a device-to-device copy of this many MiB first (HBM counter calibration)")
     return p.parse_args()
@@ -486,6 +489,31 @@ def profile_kernels(cfg, args, torch, qtx):
                       "eloc_tflops": round(n * nconn * fwd / ms_e / 1e9, 1)}))
 
 
+def relu_variant(cfg, args, torch, qtx, flops_per_sweep):
+    """The same VMC step, both modes, with the opt-in ReLU activation (the
+    reference's network uses GELU); timed like the headline."""
+    va = argparse.Namespace(**vars(args))
+    va.activation = "relu"
+    va.no_ref_solve = True
+    res = {}
+    for m in ("mixed", "plain"):
+        print(f"[bench] building {args.config} {m} (ReLU variant)", file=sys.stderr, flush=True)
+        res[m] = run_mode(cfg, m, va, torch, qtx)
+    r, p = res["mixed"], res["plain"]
+    ach = flops_per_sweep / (r["metropolis_kernel_ms"] / 1e3) / 1e12
+    out = {"activation": "relu", "value": round(r["steps_per_s"], 5), "ms_per_step": round(r["ms_per_step"], 2),
+           "stages_ms": {k: round(v, 2) for k, v in r["stages"].items()},
+           "plain": {"value": round(p["steps_per_s"], 5), "ms_per_step": round(p["ms_per_step"], 2),
+                     "stages_ms": {k: round(v, 2) for k, v in p["stages"].items()}},
+           "mixed_over_plain": round(r["steps_per_s"] / p["steps_per_s"], 3),
+           "metropolis_kernel": {"avg_launch_ms": round(r["metropolis_kernel_ms"], 3),
+                                 "achieved_tflops": round(ach, 2), "frac": round(ach / MFMA_PEAK["bf16"], 4)}}
+    if "precision" in r:
+        out["precision"] = {k: r["precision"][k] for k in ("update_vector_rel_err", "energy_rel_err",
+                                                           "logpsi_max_abs_diff") if k in r["precision"]}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -605,6 +633,9 @@ def main():
         out["eloc"] = {"kernel": "rc_eloc_kernel", "ms": round(r["eloc_ms"], 2),
                        "achieved_tflops": round(cfg["ns"] * r["nconn"] * fwd / t_e / 1e12, 1),
                        "hbm_algorithmic_bytes": nbytes, "hbm_gbps": round(nbytes / t_e / 1e9, 4)}
+    if (args.config == "c3" and args.activation == "gelu" and args.mode == "both" and
+            not args.no_relu_variant):
+        out["relu_variant"] = relu_variant(cfg, args, torch, qtx, fwd * cfg["ns"] * sweep_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds, sweep_steps)
     if not args.no_serve and args.config == "c3":
