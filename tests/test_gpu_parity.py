@@ -1,6 +1,6 @@
 """HIP path vs the CPU oracle on the same seeded inputs (needs an MI355X).
 
-Tolerances (north star): float32 network paths 1e-5 relative, float64 paths
+Tolerances: float32 network paths 1e-5 relative, float64 paths
 1e-10 relative; integer/byte work (streams, initial chains, connected
 configurations, chains of the float64 RBM) bit-exact.  Relative errors of
 log-amplitudes and local energies use an absolute floor of 1 so that values
